@@ -1,0 +1,208 @@
+/*
+ * s2d.h -- C ABI of libs2d, the MI355X (gfx950) library behind the
+ * Spectral_m / TimeIntegration_m module interfaces of mpclay/Spectral2D.
+ *
+ * Every function returns an int status (S2D_OK == 0). On failure the
+ * context's message is available from s2d_last_error(ctx) (or
+ * s2d_last_error(NULL) for errors raised before a context exists).
+ *
+ * Data contract (src/Alloc.F90:62-122, src/Spectral2D.F90:287-304, P = 1):
+ *   one field = allocLocal = cISize*nyP complex words, viewed as complex
+ *   (cISize, nyP) or real (rISize, nyP), Fortran column-major, x fastest.
+ *   Host buffers passed to s2d_set_field_host / s2d_get_field_host are
+ *   2*cISize*nyP doubles in exactly that layout (C_DOUBLE / C_DOUBLE_COMPLEX,
+ *   src/Parameters.F90:31-35). The library keeps device copies; it retains
+ *   no host pointer after a call returns.
+ *
+ * Fields are device slots named by S2D_FIELD_*. The per-routine entry
+ * points act on slots with the reference's semantics (in-place transforms,
+ * normalization placement, masks, truncation). s2d_rk3_step is the fused
+ * device-resident time step: it keeps the vorticity in a compact
+ * kept-modes-only layout between calls and refreshes the W and PSI slots
+ * only when a later call reads them.
+ *
+ * Calls on one context are synchronous with respect to the host at the
+ * ABI (results are complete when the call returns), not re-entrant, and
+ * use one HIP stream per context. Different contexts may be driven from
+ * different host threads.
+ */
+#ifndef S2D_H
+#define S2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define S2D_OK 0
+#define S2D_ERR_INVALID_ARG 1   /* bad size, field id, null pointer */
+#define S2D_ERR_UNSUPPORTED 2   /* dealiasing method or grid size not supported */
+#define S2D_ERR_HIP 3           /* HIP runtime error */
+#define S2D_ERR_ALLOC 4         /* device allocation failed */
+#define S2D_ERR_NONFINITE 5     /* NaN/Inf in the state, or CFL blow-up */
+#define S2D_ERR_NO_DEVICE 6     /* no GPU visible */
+
+/* ---- constants mirrored from the reference ---- */
+#define S2D_NO_DEALIASING 0     /* src/Spectral.F90:41 */
+#define S2D_DEALIAS_3_2_RULE 1  /* src/Spectral.F90:43 */
+#define S2D_DEALIAS_2_3_RULE 2  /* src/Spectral.F90:45 (empty in the reference: rejected) */
+#define S2D_DEALIAS_GRID_SHIFT 3 /* src/Spectral.F90:47 (empty in the reference: rejected) */
+#define S2D_RK3_TVD_SHU 1       /* src/TimeIntegration.F90:42 */
+
+/* ---- arithmetic of a context ---- */
+#define S2D_FP64 0              /* the reference's C_DOUBLE path */
+#define S2D_FP32 1              /* FP32 transforms/products/updates, FP64 twiddles and reductions */
+
+/* ---- device field slots ---- */
+#define S2D_FIELD_W 0           /* vorticity (wC / wR) */
+#define S2D_FIELD_PSI 1         /* streamfunction (psiC / psiR) */
+#define S2D_FIELD_U 2           /* u velocity / nonlinear scratch (uC / uR) */
+#define S2D_FIELD_V 3           /* v velocity / nonlinear scratch (vC / vR) */
+#define S2D_FIELD_WS 4          /* RK scratch wS (src/TimeIntegration.F90:54-58) */
+#define S2D_FIELD_PSIS 5        /* RK scratch psiS (src/TimeIntegration.F90:60-64) */
+#define S2D_FIELD_DW 6          /* RK increment dW (src/TimeIntegration.F90:52) */
+#define S2D_FIELD_TMP 7         /* free scratch for callers */
+#define S2D_NFIELDS 8
+
+typedef struct s2d_ctx s2d_ctx;
+
+/* Output of GetGridSize + GetAllocSize for one process. */
+typedef struct s2d_grid {
+    int nx, ny, dealias;
+    int nxG, nyG;
+    int kxLG, kyLG, kxMG, kyMG;
+    int kxLU, kyLU, kxMU, kyMU;
+    double kMax;
+    int rISize, cISize;
+    int nxP, nyP, j1, j2, localJ, allocLocal;
+} s2d_grid;
+
+/* FP64 diagnostics of the current vorticity (all sums over the full
+ * Hermitian spectrum, src/Analysis.F90:380-395 weights). */
+typedef struct s2d_diag {
+    double tke;         /* 1/2 <u^2 + v^2> */
+    double enstrophy;   /* 1/2 <omega^2> */
+    double palinstrophy;/* 1/2 <|grad omega|^2> */
+    double max_abs_w;   /* max |w_hat| over stored modes (the driver's print, src/Spectral2D.F90:358) */
+    double time;        /* simulation time accumulated by the step calls */
+    long long nadv;     /* steps taken by the step calls */
+} s2d_diag;
+
+/* Message of the last failure on ctx (or of a failed s2d_create when
+ * ctx is NULL). Never NULL. */
+const char *s2d_last_error(const s2d_ctx *ctx);
+
+/* GetGridSize, src/Spectral.F90:117-219, plus GetAllocSize with one
+ * process, src/Alloc.F90:62-122. Host only (no GPU needed). */
+int s2d_get_grid_size(int nx, int ny, int dealias, s2d_grid *g);
+/* kxG (cISize ints) and kyG (nyG ints) in FFTW order,
+ * src/Spectral.F90:203-218. Host only. */
+int s2d_wavenumbers(const s2d_grid *g, int *kxG, int *kyG);
+
+/* Replaces the module setup sequence GetGridSize -> SetupMask
+ * (src/Spectral.F90:229) -> SetupTruncation (:272) -> SetupPlans (:330)
+ * -> TimeIntegrationSetup (src/TimeIntegration.F90:79): allocates the
+ * device slots (zeroed), twiddle tables and transform plans.
+ * precision is S2D_FP64 or S2D_FP32. */
+int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **ctx);
+/* SpectralFinalize (src/Spectral.F90:805) + TimeIntegrationFinalize
+ * (src/TimeIntegration.F90:218). Frees everything. NULL is a no-op. */
+int s2d_destroy(s2d_ctx *ctx);
+
+int s2d_get_grid(const s2d_ctx *ctx, s2d_grid *g);
+/* SetupMask result, 0-based indices (-1 when absent). */
+int s2d_get_mask(const s2d_ctx *ctx, int *xMaskBool, int *xMaskInd,
+                 int *yMaskBool, int *yMaskInd);
+/* SetupTruncation result: 0-based first truncated i of each row (nyP
+ * ints); cISize where the reference leaves the row unset. */
+int s2d_get_truncation(const s2d_ctx *ctx, int *i1);
+
+/* Host <-> device copies of one field in the reference layout
+ * (2*cISize*nyP doubles). */
+int s2d_set_field_host(s2d_ctx *ctx, int field, const double *buf);
+int s2d_get_field_host(s2d_ctx *ctx, int field, double *buf);
+/* Device pointer of a slot (T = double for S2D_FP64, float for S2D_FP32;
+ * cISize*nyP complex T). Reading W or PSI through it after s2d_rk3_step
+ * requires s2d_sync_fields first. */
+int s2d_field_device_ptr(s2d_ctx *ctx, int field, void **ptr);
+/* Refresh the W and PSI slots from the fused step's compact state. */
+int s2d_sync_fields(s2d_ctx *ctx);
+int s2d_zero_field(s2d_ctx *ctx, int field);
+int s2d_copy_field(s2d_ctx *ctx, int src, int dst);
+
+/* TransformR2C, src/Spectral.F90:749-763: forward 2D r2c in place,
+ * then divide by nxG*nyG. */
+int s2d_transform_r2c(s2d_ctx *ctx, int field);
+/* TransformC2R, src/Spectral.F90:772-781: unnormalized 2D c2r in place
+ * (destroys the spectral input; Im of the kx = 0 and kx = nxG/2 columns is
+ * ignored). */
+int s2d_transform_c2r(s2d_ctx *ctx, int field);
+/* DuDx / DuDy, src/Spectral.F90:662-693 / 708-738 (in and out may be the
+ * same slot). */
+int s2d_dudx(s2d_ctx *ctx, int in, int out);
+int s2d_dudy(s2d_ctx *ctx, int in, int out);
+/* Truncate, src/Spectral.F90:788-802. */
+int s2d_truncate(s2d_ctx *ctx, int field);
+/* ComputePsi, src/Spectral.F90:508-560 (one process: row 1 is ky = 0). */
+int s2d_compute_psi(s2d_ctx *ctx, int w, int psi);
+/* ComputeVelocity, src/Spectral.F90:576-610. */
+int s2d_compute_velocity(s2d_ctx *ctx, int u, int v, int psi, int trans);
+/* ComputeVorticity, src/Spectral.F90:624-647 (destroys u and v). */
+int s2d_compute_vorticity(s2d_ctx *ctx, int u, int v, int w);
+/* ComputeRHS, src/Spectral.F90:396-482 (overwrites u, v; round-trips w). */
+int s2d_compute_rhs(s2d_ctx *ctx, double nu, int u, int v, int w, int psi,
+                    int dW);
+/* IntegrateOneStep, src/TimeIntegration.F90:130-210, composed literally
+ * from the per-routine kernels on slots W, PSI, U, V with scratch WS,
+ * PSIS, DW. */
+int s2d_integrate_one_step(s2d_ctx *ctx, double nu, double dt);
+
+/* The fused device-resident TVD-RK3 step (same mathematics as
+ * IntegrateOneStep; 3 kernels per stage on a compact kept-modes state).
+ * After it, W and PSI hold the new state (refreshed lazily); U and V are
+ * undefined, as after the reference step (src/Spectral2D.F90:341). */
+int s2d_rk3_step(s2d_ctx *ctx, double nu, double dt);
+/* nsteps fused steps. cfl > 0 selects a CFL time step recomputed on the
+ * device every step from the state at its start:
+ * dt = cfl * (2 pi / nx) / max over the padded grid of (|u| + |v|);
+ * cfl <= 0 uses the fixed dt. The state is checked for NaN/Inf at the
+ * end (S2D_ERR_NONFINITE). */
+int s2d_rk3_steps(s2d_ctx *ctx, int nsteps, double nu, double dt, double cfl);
+/* ComputeTimeStep, src/TimeIntegration.F90:213-215 (an empty stub in the
+ * reference): the CFL dt of the current state, computed on the device. */
+int s2d_cfl_dt(s2d_ctx *ctx, double cfl, double *dt);
+/* FP64 device reductions of the current state. */
+int s2d_diagnostics(s2d_ctx *ctx, s2d_diag *out);
+/* Simulation time / step count held by the context (advanced by the step
+ * calls); s2d_set_time resets them. */
+int s2d_set_time(s2d_ctx *ctx, double time, long long nadv);
+
+/* Seeded decaying-turbulence initial condition into W and PSI. Replaces
+ * SetTurbulentVelocity (src/SetIC.F90:125-151), whose phases come from
+ * /dev/urandom, with a reproducible recipe: kept modes 0 < |k| <= kMax,
+ * E(k) ~ k^4 exp(-(k/k0)^2), |u_k|^2 = E/(pi |k|), phase 2 pi U where U is a
+ * splitmix64 hash of (seed, kx, ky), Hermitian on kx = 0, scaled so that
+ * the ComputeSpectrum TKE (src/Analysis.F90:380-395) equals ke; PSI =
+ * ComputePsi(W). */
+int s2d_set_turbulence(s2d_ctx *ctx, unsigned long long seed, double k0, double ke);
+/* Taylor-Green vortex psi = sin x sin y, w = 2 psi sampled at x_i = i dx
+ * (1-based i), then TransformR2C: SetTaylorGreen, src/SetIC.F90:70-101. */
+int s2d_set_taylor_green(s2d_ctx *ctx);
+
+/* ---- kernel timing (benchmarks) ----
+ * Between s2d_profile_begin and s2d_profile_end every launch of the fused
+ * stage kernels is bracketed by HIP events recorded on the context's stream
+ * (up to max_launches launches). s2d_profile_end returns per-kernel total
+ * milliseconds and launch counts, indexed by S2D_KERNEL_*. No reference
+ * counterpart (the reference has no timers). */
+#define S2D_KERNEL_PASS_A 0     /* column inverse transform */
+#define S2D_KERNEL_PASS_B 1     /* row c2r . product . r2c */
+#define S2D_KERNEL_PASS_C 2     /* column forward + RHS + RK3 stage */
+#define S2D_NKERNELS 3
+int s2d_profile_begin(s2d_ctx *ctx, int max_launches);
+int s2d_profile_end(s2d_ctx *ctx, double *ms, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

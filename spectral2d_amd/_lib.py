@@ -1,0 +1,135 @@
+"""ctypes binding of libs2d.so (the C ABI declared in include/s2d.h).
+
+The library is loaded from this package directory only. There is no
+fallback: if libs2d.so is missing the import of the product API raises, so a
+run can never silently compute on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libs2d.so")
+
+# status codes (include/s2d.h)
+S2D_OK = 0
+S2D_ERR_INVALID_ARG = 1
+S2D_ERR_UNSUPPORTED = 2
+S2D_ERR_HIP = 3
+S2D_ERR_ALLOC = 4
+S2D_ERR_NONFINITE = 5
+S2D_ERR_NO_DEVICE = 6
+
+NO_DEALIASING = 0
+DEALIAS_3_2_RULE = 1
+DEALIAS_2_3_RULE = 2
+DEALIAS_GRID_SHIFT = 3
+RK3_TVD_SHU = 1
+
+FP64 = 0
+FP32 = 1
+
+FIELD_W, FIELD_PSI, FIELD_U, FIELD_V, FIELD_WS, FIELD_PSIS, FIELD_DW, FIELD_TMP = range(8)
+NFIELDS = 8
+
+# Every exported symbol of include/s2d.h.
+EXPORTS = (
+    "s2d_last_error", "s2d_get_grid_size", "s2d_wavenumbers", "s2d_create",
+    "s2d_destroy", "s2d_get_grid", "s2d_get_mask", "s2d_get_truncation",
+    "s2d_set_field_host", "s2d_get_field_host", "s2d_field_device_ptr",
+    "s2d_sync_fields", "s2d_zero_field", "s2d_copy_field", "s2d_transform_r2c",
+    "s2d_transform_c2r", "s2d_dudx", "s2d_dudy", "s2d_truncate",
+    "s2d_compute_psi", "s2d_compute_velocity", "s2d_compute_vorticity",
+    "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
+    "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_profile_begin",
+    "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
+)
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("dealias", C.c_int),
+        ("nxG", C.c_int), ("nyG", C.c_int),
+        ("kxLG", C.c_int), ("kyLG", C.c_int), ("kxMG", C.c_int), ("kyMG", C.c_int),
+        ("kxLU", C.c_int), ("kyLU", C.c_int), ("kxMU", C.c_int), ("kyMU", C.c_int),
+        ("kMax", C.c_double),
+        ("rISize", C.c_int), ("cISize", C.c_int),
+        ("nxP", C.c_int), ("nyP", C.c_int), ("j1", C.c_int), ("j2", C.c_int),
+        ("localJ", C.c_int), ("allocLocal", C.c_int),
+    ]
+
+
+class Diag(C.Structure):
+    _fields_ = [
+        ("tke", C.c_double), ("enstrophy", C.c_double), ("palinstrophy", C.c_double),
+        ("max_abs_w", C.c_double), ("time", C.c_double), ("nadv", C.c_longlong),
+    ]
+
+
+class S2DError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libs2d error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libs2d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m spectral2d_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp, dp, ip = C.c_void_p, P(C.c_double), P(C.c_int)
+    L.s2d_last_error.argtypes = [vp]
+    L.s2d_last_error.restype = C.c_char_p
+    L.s2d_get_grid_size.argtypes = [C.c_int, C.c_int, C.c_int, P(Grid)]
+    L.s2d_wavenumbers.argtypes = [P(Grid), ip, ip]
+    L.s2d_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]
+    L.s2d_destroy.argtypes = [vp]
+    L.s2d_get_grid.argtypes = [vp, P(Grid)]
+    L.s2d_get_mask.argtypes = [vp, ip, ip, ip, ip]
+    L.s2d_get_truncation.argtypes = [vp, ip]
+    L.s2d_set_field_host.argtypes = [vp, C.c_int, dp]
+    L.s2d_get_field_host.argtypes = [vp, C.c_int, dp]
+    L.s2d_field_device_ptr.argtypes = [vp, C.c_int, P(vp)]
+    L.s2d_sync_fields.argtypes = [vp]
+    L.s2d_zero_field.argtypes = [vp, C.c_int]
+    L.s2d_copy_field.argtypes = [vp, C.c_int, C.c_int]
+    L.s2d_transform_r2c.argtypes = [vp, C.c_int]
+    L.s2d_transform_c2r.argtypes = [vp, C.c_int]
+    L.s2d_dudx.argtypes = [vp, C.c_int, C.c_int]
+    L.s2d_dudy.argtypes = [vp, C.c_int, C.c_int]
+    L.s2d_truncate.argtypes = [vp, C.c_int]
+    L.s2d_compute_psi.argtypes = [vp, C.c_int, C.c_int]
+    L.s2d_compute_velocity.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.s2d_compute_vorticity.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.s2d_compute_rhs.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.s2d_integrate_one_step.argtypes = [vp, C.c_double, C.c_double]
+    L.s2d_rk3_step.argtypes = [vp, C.c_double, C.c_double]
+    L.s2d_rk3_steps.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.s2d_cfl_dt.argtypes = [vp, C.c_double, dp]
+    L.s2d_diagnostics.argtypes = [vp, P(Diag)]
+    L.s2d_set_time.argtypes = [vp, C.c_double, C.c_longlong]
+    L.s2d_profile_begin.argtypes = [vp, C.c_int]
+    L.s2d_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
+    L.s2d_set_turbulence.argtypes = [vp, C.c_ulonglong, C.c_double, C.c_double]
+    L.s2d_set_taylor_green.argtypes = [vp]
+    for name in EXPORTS:
+        if name != "s2d_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None):
+    if rc != S2D_OK:
+        msg = lib().s2d_last_error(ctx)
+        raise S2DError(rc, msg.decode() if msg else "")

@@ -1,0 +1,79 @@
+// s2d_internal.hpp -- context and shared declarations of libs2d.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fft_block.hpp"
+#include "s2d.h"
+
+namespace s2d {
+
+// Elements per thread for a length-n transform: the largest of
+// 12, 8, 6, 4, 3, 2 that divides n and whose radices (a subset of
+// {8, 4, 3, 2} dividing E) can factor n.
+int choose_elems(int n);
+// Radix plan for n with E elements per thread; false when n is not 2^a 3^b.
+bool make_plan(int n, int E, FftPlan *p);
+
+// A transform length with its plan and device twiddle table
+// tw[e] = exp(-2 pi i e / n) (FP64 generation, stored as T).
+struct DevFft {
+    FftPlan plan{};
+    int E = 0;
+    void *tw = nullptr;
+};
+
+// Device scalars of a context (FP64 / integer, one small buffer).
+struct DevScalars {
+    unsigned long long vmax_bits; // max over the padded grid of |u|+|v| (double bits)
+    double dt;                    // dt of the last step
+    double time;                  // accumulated simulation time
+    long long nadv;               // accumulated steps
+    double pad[4];
+};
+
+} // namespace s2d
+
+struct s2d_ctx {
+    s2d_grid g{};
+    int precision = S2D_FP64;
+    size_t celt = 16; // bytes of one complex element on the device
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int max_lds = 65536;
+
+    std::vector<int> kx, ky, trunc_i1;
+    int xMaskBool = 0, xMaskInd = -1, yMaskBool = 0, yMaskInd = -1;
+
+    void *field[S2D_NFIELDS] = {};
+    int *d_trunc = nullptr; // per row: first truncated i (0-based)
+    int *d_kx = nullptr;
+    int *d_ky = nullptr;
+
+    s2d::DevFft col;  // length nyG complex (columns)
+    s2d::DevFft rowh; // length nxG/2 complex (rows, half-length real trick)
+    s2d::DevFft rowf; // length nxG complex (rows, two-for-one in the fused pass)
+    void *tw_x = nullptr; // exp(-2 pi i k / nxG), k = 0..nxG-1 (= rowf.tw)
+
+    // fused compact state
+    bool fused = false;
+    int N = 0, K = 0, KP = 0; // resolved size, kept kx count, compact column pitch
+    void *Wn = nullptr, *Ws = nullptr;
+    void *G[3] = {}, *H[2] = {};
+    s2d::DevScalars *d_sc = nullptr;
+    double *d_red = nullptr; // reduction partials
+    int n_red = 0;
+    bool ref_stale = false;     // W/PSI slots behind the compact state
+    bool compact_stale = true;  // compact state behind the W slot
+
+    // kernel timing (s2d_profile_begin / _end)
+    bool prof_on = false;
+    int prof_used = 0;
+    std::vector<hipEvent_t> prof_ev; // 2 per launch
+    std::vector<int> prof_kind;
+
+    std::string err;
+};

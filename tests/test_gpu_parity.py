@@ -1,0 +1,241 @@
+"""GPU parity of libs2d against the CPU FP64 oracle (reference layout).
+
+Each case prints what it runs before touching the GPU. Tolerances:
+pointwise kernels are bitwise (the same expression order, no contraction);
+transforms and everything built on them are compared in relative L2 / max
+norm at the FP64 rounding level (<= 1e-13 per transform, <= 1e-12 after
+multi-step runs, the bar stated for the drop-in).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from spectral2d_amd.spectral import Spectral2D
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [8, 16, 24, 32, 64, 96, 128]
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
+
+
+def _pair(N, seed=0):
+    print(f"[parity] N={N} seed={seed}", flush=True)
+    o = oracle.Oracle(N)
+    s = Spectral2D(N)
+    return o, s
+
+
+def _random_spectral(o, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal((o.nyG, o.cISize)) + 1j * rng.standard_normal((o.nyG, o.cISize))
+    return o.from_complex(c)
+
+
+def _turbulence(o, seed=1, k0=None):
+    w, psi = o.field(), o.field()
+    o.set_turbulence(w, psi, seed=seed, k0=k0 if k0 else max(2.0, o.nx / 8), ke=0.5)
+    return w, psi
+
+
+@pytest.mark.parametrize("N", SIZES)
+def test_transforms(N):
+    o, s = _pair(N)
+    rng = np.random.default_rng(N)
+    # c2r of a random spectral field (the Im of kx=0 / Nyquist are ignored by both)
+    buf = _random_spectral(o, N)
+    ref = buf.copy()
+    o.transform_c2r(ref)
+    s.set_field("tmp", buf)
+    s.TransformC2R("tmp")
+    got = s.get_field("tmp")
+    r_ref = o.rview(ref)[:, : o.nxG]
+    r_got = s.rview(got)[:, : o.nxG]
+    assert rel_l2(r_got, r_ref) < 1e-14 * max(1, np.log2(o.nxG * o.nyG))
+    # r2c of a random real field
+    real = rng.standard_normal((o.nyG, o.nxG))
+    buf = o.from_real(real)
+    ref = buf.copy()
+    o.transform_r2c(ref)
+    s.set_field("tmp", buf)
+    s.TransformR2C("tmp")
+    got = s.get_field("tmp")
+    assert rel_l2(s.cview(got), o.cview(ref)) < 1e-14 * max(1, np.log2(o.nxG * o.nyG))
+    s.close()
+
+
+@pytest.mark.parametrize("N", [8, 32, 96])
+def test_pointwise_bitwise(N):
+    o, s = _pair(N)
+    a = _random_spectral(o, 7)
+    b = _random_spectral(o, 8)
+    # DuDx / DuDy
+    for name, fo, fs in (("dudx", o.dudx, s.DuDx), ("dudy", o.dudy, s.DuDy)):
+        ref = o.field()
+        fo(a, ref)
+        s.set_field("u", a)
+        fs("u", "v")
+        assert np.array_equal(s.get_field("v"), ref), name
+    # Truncate
+    ref = a.copy()
+    o.truncate(ref)
+    s.set_field("tmp", a)
+    s.Truncate("tmp")
+    assert np.array_equal(s.get_field("tmp"), ref)
+    # ComputePsi
+    ref = o.field()
+    o.compute_psi(a, ref)
+    s.set_field("w", a)
+    s.ComputePsi("w", "psi")
+    assert np.array_equal(s.get_field("psi"), ref)
+    # ComputeVelocity (spectral)
+    u, v = o.field(), o.field()
+    o.compute_velocity(u, v, a, trans=False)
+    s.set_field("psi", a)
+    s.ComputeVelocity("u", "v", "psi", trans=False)
+    assert np.array_equal(s.get_field("u"), u)
+    assert np.array_equal(s.get_field("v"), v)
+    # ComputeVorticity (destroys u, v)
+    uo, vo, wo = a.copy(), b.copy(), o.field()
+    o.compute_vorticity(uo, vo, wo)
+    s.set_field("u", a)
+    s.set_field("v", b)
+    s.ComputeVorticity("u", "v", "w")
+    assert np.array_equal(s.get_field("w"), wo)
+    assert np.array_equal(s.get_field("u"), uo)
+    assert np.array_equal(s.get_field("v"), vo)
+    s.close()
+
+
+@pytest.mark.parametrize("N", [16, 64])
+def test_compute_rhs(N):
+    o, s = _pair(N)
+    w, psi = _turbulence(o)
+    u, v, dW = o.field(), o.field(), o.field()
+    wo = w.copy()
+    o.compute_rhs(1e-3, u, v, wo, psi, dW)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.ComputeRHS(1e-3)
+    assert rel_l2(s.cview(s.get_field("dW")), o.cview(dW)) < 1e-13
+    # side effect: w round-tripped
+    assert rel_l2(s.cview(s.get_field("w")), o.cview(wo)) < 1e-14
+    s.close()
+
+
+def _run_pair(N, steps, nu, fused, seed=1, cfl=0.5):
+    o, s = _pair(N, seed)
+    w, psi = _turbulence(o, seed)
+    dt = o.cfl_dt(psi, cfl)
+    st = o.new_state()
+    st["w"][:] = w
+    st["psi"][:] = psi
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    for _ in range(steps):
+        o.step(st, nu, dt)
+        if not fused:
+            s.IntegrateOneStep(nu, dt)
+    if fused:
+        s.rk3_steps(steps, nu, dt, 0.0)
+    return o, s, st
+
+
+@pytest.mark.parametrize("N,steps", [(16, 20), (64, 20), (96, 10)])
+def test_integrate_one_step_literal(N, steps):
+    o, s, st = _run_pair(N, steps, 1e-3, fused=False)
+    e = rel_l2(s.cview(s.get_field("w")), o.cview(st["w"]))
+    print(f"  literal N={N} steps={steps} rel_l2(w)={e:.3e}")
+    assert e < 1e-12
+    assert rel_l2(s.cview(s.get_field("psi")), o.cview(st["psi"])) < 1e-12
+    s.close()
+
+
+@pytest.mark.parametrize("N,steps", [(16, 20), (32, 50), (64, 50), (96, 20), (128, 20)])
+def test_fused_rk3(N, steps):
+    o, s, st = _run_pair(N, steps, 1e-3, fused=True)
+    e = rel_l2(s.cview(s.get_field("w")), o.cview(st["w"]))
+    print(f"  fused N={N} steps={steps} rel_l2(w)={e:.3e}")
+    assert e < 1e-12
+    assert rel_l2(s.cview(s.get_field("psi")), o.cview(st["psi"])) < 1e-12
+    # truncated modes are exactly zero, as after the reference's Truncate
+    keep = o.kept_mask()
+    assert np.all(s.cview(s.get_field("w"))[~keep] == 0)
+    s.close()
+
+
+def test_fused_equals_literal_path():
+    N = 48
+    print(f"[parity] fused vs literal N={N}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = _turbulence(o, 3)
+    a, b = Spectral2D(N), Spectral2D(N)
+    for x in (a, b):
+        x.set_field("w", w)
+        x.set_field("psi", psi)
+    for _ in range(10):
+        a.IntegrateOneStep(1e-3, 2e-3)
+    b.rk3_steps(10, 1e-3, 2e-3)
+    assert rel_l2(b.cview(b.get_field("w")), a.cview(a.get_field("w"))) < 1e-13
+    a.close()
+    b.close()
+
+
+def test_taylor_green_decay():
+    """C0: 256^2 (384^2 padded), nu=0.01, dt=1e-3, 1000 steps vs the
+    analytic omega0 exp(-2 nu t) (src/SetIC.F90:47-56)."""
+    N, nu, dt, steps = 256, 0.01, 1e-3, 1000
+    print(f"[parity] Taylor-Green N={N} steps={steps}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = o.field(), o.field()
+    o.set_taylor_green(w, psi)
+    s = Spectral2D(N)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.rk3_steps(steps, nu, dt)
+    got = s.cview(s.get_field("w"))
+    exact = o.cview(w) * np.exp(-2 * nu * steps * dt)
+    e = rel_l2(got, exact)
+    print(f"  TG rel_l2={e:.3e}")
+    assert e < 1e-12
+    d = s.diagnostics()
+    assert abs(d["time"] - steps * dt) < 1e-12 and d["nadv"] == steps
+    s.close()
+
+
+@pytest.mark.slow
+def test_c1_parity_100_steps():
+    """C1 FP64 drop-in: N=1024 (1536^2 padded), seeded E(k)~k^4 exp(-(k/10)^2),
+    KE=0.5, nu=1e-4, CFL 0.5 dt; 100 steps; <= 1e-12 relative L2 in w."""
+    import os
+    N, nu, steps = 1024, 1e-4, 100
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    print(f"[parity] C1 N={N} steps={steps}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = o.field(), o.field()
+    o.set_turbulence(w, psi, seed=1, k0=10.0, ke=0.5)
+    st = o.new_state()
+    st["w"][:] = w
+    st["psi"][:] = psi
+    dts = []
+    for k in range(steps):
+        dt = o.cfl_dt(st["psi"], 0.5)
+        dts.append(dt)
+        o.step(st, nu, dt)
+        if k % 20 == 0:
+            print(f"  oracle step {k}", flush=True)
+    s = Spectral2D(N)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.rk3_steps(steps, nu, 0.0, 0.5)
+    d = s.diagnostics()
+    e = rel_l2(s.cview(s.get_field("w")), o.cview(st["w"]))
+    print(f"  C1 rel_l2(w)={e:.3e} time gpu={d['time']:.15e} cpu={sum(dts):.15e}")
+    assert abs(d["time"] - sum(dts)) < 1e-12 * sum(dts)
+    assert e < 1e-12
+    s.close()
