@@ -43,6 +43,68 @@ def alg_bytes_per_launch(N: int, s: int):
     return (c * K * (N + 3 * Np), c * K * (5 * Np), c * K * (2 * Np + 3 * N))
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
+KERNEL_TAGS = ("k_passA", "k_passB", "k_passC")
+
+
+def kernel_source_hash() -> str:
+    """Hash of the HIP sources: measured traffic is only reported for the
+    kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "spectral2d_amd", "csrc")
+    for fn in sorted(os.listdir(d)):
+        with open(os.path.join(d, fn), "rb") as f:
+            h.update(fn.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, copy_mib: int, N: int, prec: str) -> dict:
+    """Per-dispatch HBM bytes of the fused kernels from two rocprofv3 --pmc
+    runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the
+    counter reads for the plain copy of copy_mib MiB in the same run."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        (path,) = glob.glob(os.path.join(d, "*counter_collection.csv"))
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]
+                  or "copy" in r["Kernel_Name"].lower()]
+        factor = (copy_mib * 1024.0) / max(copies)
+        per = {}
+        for tag in KERNEL_TAGS:
+            v = [float(r["Counter_Value"]) for r in rows if f"{tag}<" in r["Kernel_Name"]]
+            per[tag] = sum(v) / len(v) * 1024.0 * factor if v else None
+        return per, factor
+
+    fetch, ff = load(fetch_dir, "FETCH_SIZE")
+    write, wf = load(write_dir, "WRITE_SIZE")
+    out = {"source_hash": kernel_source_hash(), "N": N, "precision": prec,
+           "fetch_factor": ff, "write_factor": wf, "copy_mib": copy_mib,
+           "kernels": {t: {"read_bytes": fetch[t], "write_bytes": write[t],
+                           "traffic": (fetch[t] + write[t]) if fetch[t] and write[t] else None}
+                       for t in KERNEL_TAGS}}
+    os.makedirs(os.path.dirname(TRAFFIC_JSON), exist_ok=True)
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def measured_traffic(N: int, prec: str):
+    """Per-kernel traffic from profiles/traffic.json when it was measured on
+    these kernels and this workload; None otherwise."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("source_hash") != kernel_source_hash() or t.get("N") != N or \
+            t.get("precision") != prec:
+        return None
+    return {k: v.get("traffic") for k, v in t["kernels"].items()}
+
+
 def cpu_baseline(N: int, nu: float, cfl: float, budget_s: float):
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -82,7 +144,20 @@ def main(argv=None):
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU-oracle stepping for cpu_baseline (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (torch) so a counter "
+                         "profile of this command holds a copy kernel of known size")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="parse two rocprofv3 --pmc output directories (FETCH_SIZE, "
+                         "WRITE_SIZE runs made with --copy-calibration) into "
+                         "profiles/traffic.json and exit")
     args = ap.parse_args(argv)
+
+    if args.pmc_summary:
+        out = pmc_summary(args.pmc_summary[0], args.pmc_summary[1],
+                          args.copy_calibration or 256, args.n, args.precision)
+        print(json.dumps(out, indent=1))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported here "
@@ -90,6 +165,16 @@ def main(argv=None):
         return 2
 
     from spectral2d_amd.spectral import FP32, FP64, Spectral2D
+
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copy calibration: 3 x {args.copy_calibration} MiB read + write", file=sys.stderr)
 
     N = args.n
     prec = FP64 if args.precision == "fp64" else FP32
@@ -116,12 +201,15 @@ def main(argv=None):
     avg_ms = [ms / max(n, 1) for ms, n in prof]
     dom = max(range(3), key=lambda k: prof[k][0])
     achieved = per[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(N, args.precision) or {}
     kernel_table = {
         KERNELS[k]: {"avg_us": round(avg_ms[k] * 1e3, 3), "launches": prof[k][1],
                      "alg_bytes": per[k],
-                     "gbs": round(per[k] / (avg_ms[k] * 1e-3) / 1e9, 1) if avg_ms[k] else None}
+                     "gbs": round(per[k] / (avg_ms[k] * 1e-3) / 1e9, 1) if avg_ms[k] else None,
+                     "traffic": traffic.get(KERNEL_TAGS[k])}
         for k in range(3)
     }
+    dom_traffic = traffic.get(KERNEL_TAGS[dom])
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -141,7 +229,8 @@ def main(argv=None):
                    "sim_time": diag["time"]},
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(dom_traffic) if dom_traffic else None},
         "kernels": kernel_table,
     }
     if args.cpu_budget > 0:

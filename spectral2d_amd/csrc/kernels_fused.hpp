@@ -63,7 +63,10 @@ __global__ void k_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem);
     const int t = threadIdx.x % p.T, f = threadIdx.x / p.T;
     const int kx = blockIdx.x;
-    if (reset_max && kx == 0 && threadIdx.x == 0) sc->vmax_bits = 0ull;
+    if (reset_max && kx == 0 && threadIdx.x == 0) {
+        sc->vmax_bits = 0ull;
+        sc->bad = 0u;
+    }
     const cpx<T> *src = Wsrc + (size_t)kx * d.KP;
     const T qx = (T)kx;
     cpx<T> v[E];
@@ -133,7 +136,8 @@ template <typename T, int E>
 __global__ __launch_bounds__(512) void k_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
                         const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw,
                         cpx<T> *__restrict__ Hvw, FusedDims d, FftPlan p,
-                        const cpx<T> *__restrict__ tw, DevScalars *sc, int B, int forward)
+                        const cpx<T> *__restrict__ tw, DevScalars *sc, int B, int forward,
+                        int track_max)
 {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem[];
@@ -197,7 +201,9 @@ __global__ __launch_bounds__(512) void k_passB(const cpx<T> *__restrict__ Gu, co
     }
 
     // max of |u|+|v| over the block -> device max (positive doubles order
-    // like their bit patterns; NaN bits sort above +Inf)
+    // like their bit patterns; NaN bits sort above +Inf). Only the first
+    // stage of a step reports it: every stage's dt derives from that value.
+    if (!track_max) return;
     double dm = (double)vmax;
     if (dm != dm) dm = __longlong_as_double(0x7ff8000000000000ll);
     for (int off = 32; off > 0; off >>= 1) {
@@ -288,6 +294,19 @@ __global__ void k_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict
         sc->time = sc->time + dtd;
         sc->nadv = sc->nadv + 1;
     }
+}
+
+// Flags a non-finite value anywhere in a complex array (sc->bad).
+template <typename T>
+__global__ void k_check_finite(const cpx<T> *__restrict__ a, size_t n, DevScalars *sc)
+{
+    bool ok = true;
+    S2D_GRID_LOOP(k, n)
+    {
+        const cpx<T> x = a[k];
+        ok = ok && (x.re - x.re == (T)0) && (x.im - x.im == (T)0);
+    }
+    if (!ok) atomicOr(&sc->bad, 1u);
 }
 
 // ---------------------------------------------------- pack / unpack

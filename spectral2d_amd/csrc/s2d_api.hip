@@ -529,7 +529,7 @@ int launch_passA(s2d_ctx *c, const cpx<T> *src, int reset)
 }
 
 template <typename T, int E>
-int launch_passB(s2d_ctx *c, int forward)
+int launch_passB(s2d_ctx *c, int forward, int track_max)
 {
     ProfScope ps(c, S2D_KERNEL_PASS_B);
     FusedDims d = fdims(c);
@@ -541,7 +541,8 @@ int launch_passB(s2d_ctx *c, int forward)
     const int blocks = (d.Np + 2 * B - 1) / (2 * B);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(B * p.T), lds, c->stream, (const cpx<T> *)c->G[0],
                        (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0],
-                       (cpx<T> *)c->H[1], d, p, (const cpx<T> *)c->rowf.tw, c->d_sc, B, forward);
+                       (cpx<T> *)c->H[1], d, p, (const cpx<T> *)c->rowf.tw, c->d_sc, B, forward,
+                       track_max);
     S2D_LAUNCHED(c, "k_passB");
     return S2D_OK;
 }
@@ -578,7 +579,7 @@ int fused_steps_E(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
             cpx<T> *dst = stage == 3 ? Wn : Ws;
             a.stage = stage;
             S2D_TRY((launch_passA<T, EC>(c, src, stage == 1)));
-            S2D_TRY((launch_passB<T, ER>(c, 1)));
+            S2D_TRY((launch_passB<T, ER>(c, 1, stage == 1)));
             S2D_TRY((launch_passC<T, EC>(c, src, dst, a)));
         }
     }
@@ -609,12 +610,12 @@ int fused_vmax(s2d_ctx *c, double *vmax)
     int r = S2D_ERR_UNSUPPORTED;
     const cpx<T> *Wn = (const cpx<T> *)c->Wn;
     switch (EC) {
-    case 12: r = launch_passA<T, 12>(c, Wn, 1); if (!r) r = launch_passB<T, 12>(c, 0); break;
-    case 8: r = launch_passA<T, 8>(c, Wn, 1); if (!r) r = launch_passB<T, 8>(c, 0); break;
-    case 6: r = launch_passA<T, 6>(c, Wn, 1); if (!r) r = launch_passB<T, 6>(c, 0); break;
-    case 4: r = launch_passA<T, 4>(c, Wn, 1); if (!r) r = launch_passB<T, 4>(c, 0); break;
-    case 3: r = launch_passA<T, 3>(c, Wn, 1); if (!r) r = launch_passB<T, 3>(c, 0); break;
-    case 2: r = launch_passA<T, 2>(c, Wn, 1); if (!r) r = launch_passB<T, 2>(c, 0); break;
+    case 12: r = launch_passA<T, 12>(c, Wn, 1); if (!r) r = launch_passB<T, 12>(c, 0, 1); break;
+    case 8: r = launch_passA<T, 8>(c, Wn, 1); if (!r) r = launch_passB<T, 8>(c, 0, 1); break;
+    case 6: r = launch_passA<T, 6>(c, Wn, 1); if (!r) r = launch_passB<T, 6>(c, 0, 1); break;
+    case 4: r = launch_passA<T, 4>(c, Wn, 1); if (!r) r = launch_passB<T, 4>(c, 0, 1); break;
+    case 3: r = launch_passA<T, 3>(c, Wn, 1); if (!r) r = launch_passB<T, 3>(c, 0, 1); break;
+    case 2: r = launch_passA<T, 2>(c, Wn, 1); if (!r) r = launch_passB<T, 2>(c, 0, 1); break;
     default: return fail(c, S2D_ERR_UNSUPPORTED, "no fused kernel for E=%d", EC);
     }
     if (r) return r;
@@ -1121,13 +1122,24 @@ int s2d_rk3_steps(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
     if (nsteps == 0) return sync(c);
     S2D_TRY(S2D_DISPATCH_T(c, fused_steps, nsteps, nu, dt, cfl));
     c->ref_stale = true;
+    {
+        const size_t n = (size_t)c->K * c->KP;
+        if (c->precision == S2D_FP64)
+            hipLaunchKernelGGL(k_check_finite<double>, dim3(grid_blocks(n, 256)), dim3(256), 0,
+                               c->stream, (const cpx<double> *)c->Wn, n, c->d_sc);
+        else
+            hipLaunchKernelGGL(k_check_finite<float>, dim3(grid_blocks(n, 256)), dim3(256), 0,
+                               c->stream, (const cpx<float> *)c->Wn, n, c->d_sc);
+        S2D_LAUNCHED(c, "k_check_finite");
+    }
     DevScalars sc;
     S2D_HIP(c, hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
     S2D_TRY(sync(c));
     double vmax;
     std::memcpy(&vmax, &sc.vmax_bits, sizeof(double));
-    if (!(vmax == vmax) || vmax > 1.0e300)
-        return fail(c, S2D_ERR_NONFINITE, "non-finite velocity in the state (max |u|+|v| = %g)",
+    if (sc.bad || !(vmax == vmax) || vmax > 1.0e300)
+        return fail(c, S2D_ERR_NONFINITE,
+                    "non-finite values in the state (max |u|+|v| at the last step start = %g)",
                     vmax);
     if (cfl > 0.0 && !(sc.dt > 0.0 && sc.dt < 1.0e300))
         return fail(c, S2D_ERR_NONFINITE, "CFL time step collapsed (dt = %g)", sc.dt);

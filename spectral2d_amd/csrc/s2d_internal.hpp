@@ -32,7 +32,9 @@ struct DevScalars {
     double dt;                    // dt of the last step
     double time;                  // accumulated simulation time
     long long nadv;               // accumulated steps
-    double pad[4];
+    unsigned int bad;             // nonzero: a non-finite value was seen
+    unsigned int pad32;
+    double pad[3];
 };
 
 } // namespace s2d

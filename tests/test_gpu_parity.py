@@ -169,6 +169,50 @@ def test_fused_rk3(N, steps):
     s.close()
 
 
+@pytest.mark.parametrize("N,steps", [(32, 40), (64, 30)])
+def test_fused_cfl_mode(N, steps):
+    """Device CFL dt (cfl * 2pi/N / max|u|+|v|, from the state at each step's
+    start) against the oracle recomputing the same dt on the CPU."""
+    print(f"[parity] CFL mode N={N} steps={steps}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = _turbulence(o, 4)
+    st = o.new_state()
+    st["w"][:] = w
+    st["psi"][:] = psi
+    total = 0.0
+    for _ in range(steps):
+        dt = o.cfl_dt(st["psi"], 0.5)
+        total += dt
+        o.step(st, 1e-3, dt)
+    s = Spectral2D(N)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.set_time(0.0, 0)
+    s.rk3_steps(steps, 1e-3, 0.0, 0.5)
+    d = s.diagnostics()
+    assert d["nadv"] == steps
+    assert abs(d["time"] - total) < 1e-13 * total
+    assert rel_l2(s.cview(s.get_field("w")), o.cview(st["w"])) < 1e-12
+    # ComputeTimeStep of the final state matches the oracle's
+    assert abs(s.ComputeTimeStep(0.5) - o.cfl_dt(st["psi"], 0.5)) < 1e-13 * total
+    s.close()
+
+
+def test_nonfinite_state_is_reported():
+    from spectral2d_amd import S2DError
+    N = 16
+    print(f"[parity] NaN injection N={N}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = _turbulence(o, 2)
+    s = Spectral2D(N)
+    w[2 * (3 + 2 * o.cISize)] = np.nan
+    s.set_field("w", w)
+    with pytest.raises(S2DError) as e:
+        s.rk3_steps(2, 1e-3, 1e-3)
+    assert e.value.code == 5
+    s.close()
+
+
 def test_fused_equals_literal_path():
     N = 48
     print(f"[parity] fused vs literal N={N}", flush=True)
