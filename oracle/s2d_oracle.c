@@ -11,6 +11,7 @@
  * recursive decimation-in-time mixed-radix (4, 2, 3) transform; the
  * reference used FFTW 3.3.3, which is not available.
  */
+#define _GNU_SOURCE /* sincos */
 #include "s2d_oracle.h"
 
 #include <math.h>
@@ -709,8 +710,12 @@ void or_set_turbulence(const or_grid_t *g, const int *kxP, const int *kyP,
             double th = 2.0 * PI * or_uniform(seed, sx, sy);
             double amp = au * km;            /* |w_k| = |k| |u_k| */
             size_t k = 2 * ((size_t)i + (size_t)j * cI);
-            w[k] = amp * cos(th);
-            w[k + 1] = conj ? -(amp * sin(th)) : amp * sin(th);
+            /* glibc sincos, called explicitly so that any compiler's build
+             * of this recipe gives the same bits */
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            w[k] = amp * cs;
+            w[k + 1] = conj ? -(amp * sn) : amp * sn;
             double wt = (kx == 0 || kx == g->kxMG) ? 1.0 : 2.0;
             tke += wt * 0.5 * au * au;
         }

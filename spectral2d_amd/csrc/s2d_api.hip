@@ -1266,8 +1266,10 @@ int s2d_set_turbulence(s2d_ctx *c, unsigned long long seed, double k0, double ke
             const double th = 2.0 * PI * hash_uniform(seed, kx, sy);
             const double amp = au * km; // |w_k|
             const size_t k = 2 * ((size_t)i + (size_t)j * cI);
-            w[k] = amp * std::cos(th);
-            w[k + 1] = conj ? -(amp * std::sin(th)) : amp * std::sin(th);
+            double sn, cs; // glibc sincos: the same bits whichever compiler builds this
+            ::sincos(th, &sn, &cs);
+            w[k] = amp * cs;
+            w[k + 1] = conj ? -(amp * sn) : amp * sn;
             const double wt = (kx == 0 || kx == c->g.kxMG) ? 1.0 : 2.0;
             tke += wt * 0.5 * au * au;
         }
