@@ -1,0 +1,273 @@
+// kernels_fused_v2.hpp -- the fused RK3 stage kernels on wave-level FFTs,
+// compiled for fixed padded sizes Np = 24 * TG (TG = 4..64 lanes per
+// transform) and, in FP32, Np = 3072 = 48 * 64. Same mathematics and data
+// layout as kernels_fused.hpp (compact omega, mixed (kx, y) fields G/H);
+// other sizes use those kernels.
+//
+//  A  one lane group per (kx, field): u_hat, v_hat or w_hat of a column,
+//     inverse transform along y.
+//  B  one lane group per row y: w from its Hermitian spectrum, u + i v
+//     (two-for-one), the products (u w + i v w) in registers, one forward
+//     transform, the uw / vw spectra separated from P_k and conj P_{Np-k}.
+//     Rows are assigned XCD-major so that the rows in flight on one XCD are
+//     adjacent and share the 128-byte lines of the (kx, y) fields in its L2.
+//  C  two waves per column group: one transforms uw, the other vw; they swap
+//     halves of their results through LDS and finish the RHS, truncation and
+//     Shu-Osher stage update on the compact state.
+#pragma once
+
+#include "fft_wave.hpp"
+#include "kernels_fused.hpp"
+
+namespace s2d {
+
+template <typename T, int NP>
+struct WaveCfg {
+    static constexpr int E = (NP % 48 == 0 && NP / 48 == 64) ? 48 : 24;
+    static constexpr int TG = NP / E;
+    static constexpr int GPW = 64 / TG; // transforms per wave
+    static constexpr int SLICE = LdsPad<T>::size(NP);
+    // register budget: at least this many waves per SIMD
+    static constexpr int WPE = (sizeof(T) == 8 || E == 48) ? 2 : 3;
+    static constexpr int WPE_B = (sizeof(T) == 8 || E == 48) ? 1 : 2; // pass B holds w and u + i v
+    static_assert(E * TG == NP && TG >= 2 && TG <= 64 && (TG & (TG - 1)) == 0,
+                  "unsupported wave FFT size");
+};
+
+// XCD-major block order: the hardware deals blocks round-robin to 8 XCDs,
+// so block b runs on XCD b % 8; give each XCD a contiguous range of work.
+__device__ __forceinline__ int xcd_major(int b, int nblocks)
+{
+    if (nblocks % 8 != 0) return b; // the permutation below needs equal shares
+    const int per = nblocks / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// ------------------------------------------------------------- pass A
+template <typename T, int NP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void k2_passA(const cpx<T> *__restrict__ Wsrc,
+                                               cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
+                                               cpx<T> *__restrict__ Gw, FusedDims d,
+                                               const cpx<T> *__restrict__ tw, DevScalars *sc,
+                                               int reset_max)
+{
+#pragma clang fp contract(off)
+    using C = WaveCfg<T, NP>;
+    constexpr int E = C::E, TG = C::TG;
+    __shared__ cpx<T> lds[C::GPW * C::SLICE];
+    const int g = threadIdx.x / TG, t = threadIdx.x % TG;
+    const int job = blockIdx.x * C::GPW + g;
+    const bool active = job < 3 * d.K;
+    const int kx = active ? job / 3 : 0, f = job % 3;
+    if (reset_max && blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->vmax_bits = 0ull;
+        sc->bad = 0u;
+    }
+    const cpx<T> *src = Wsrc + (size_t)kx * d.KP;
+    const T qx = (T)kx;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int j = t + TG * s;
+        const int c = compact_row(j, d);
+        cpx<T> w = (active && c >= 0) ? src[c] : mk<T>((T)0, (T)0);
+        const int kyi = (j <= NP / 2) ? j : j - NP;
+        const T qy = (T)kyi;
+        if (f == 2) {
+            v[s] = w;
+        } else {
+            cpx<T> ps = mk<T>((T)0, (T)0);
+            if (!(kx == 0 && kyi == 0)) {
+                const T dd = (kx == 0) ? qy * qy : (kyi == 0) ? qx * qx : qx * qx + qy * qy;
+                ps = mk<T>(w.re / dd, w.im / dd);
+            }
+            if (f == 0) {
+                v[s] = mk<T>(-(qy * ps.im), qy * ps.re); // II*ky*psi
+            } else {
+                cpx<T> r = mk<T>(-(qx * ps.im), qx * ps.re); // II*kx*psi
+                v[s] = mk<T>((T)-1 * r.re, (T)-1 * r.im);
+            }
+        }
+    }
+    wave_fft<T, E, TG, 1>(v, lds + g * C::SLICE, t, tw);
+    if (active) {
+        cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + (size_t)kx * NP;
+#pragma unroll
+        for (int s = 0; s < E; ++s) dst[t + TG * s] = v[s];
+    }
+}
+
+// ------------------------------------------------------------- pass B
+
+// Hermitian extension of one row's x spectrum at frequency k of a length-NP
+// transform (only Re of k = 0 is read).
+template <typename T>
+__device__ __forceinline__ cpx<T> hermitian_at(const cpx<T> *__restrict__ G, int k, int y,
+                                               const FusedDims &d)
+{
+    const int h = d.N / 2;
+    if (k <= h) {
+        cpx<T> a = G[(size_t)k * d.Np + y];
+        if (k == 0) a.im = (T)0;
+        return a;
+    }
+    if (k >= d.Np - h) {
+        cpx<T> a = G[(size_t)(d.Np - k) * d.Np + y];
+        a.im = -a.im;
+        return a;
+    }
+    return mk<T>((T)0, (T)0);
+}
+
+template <typename T, int NP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void k2_passB(const cpx<T> *__restrict__ Gu,
+                                               const cpx<T> *__restrict__ Gv,
+                                               const cpx<T> *__restrict__ Gw,
+                                               cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
+                                               FusedDims d, const cpx<T> *__restrict__ tw,
+                                               DevScalars *sc, int forward, int track_max)
+{
+#pragma clang fp contract(off)
+    using C = WaveCfg<T, NP>;
+    constexpr int E = C::E, TG = C::TG;
+    __shared__ cpx<T> lds_all[C::GPW * C::SLICE];
+    const int g = threadIdx.x / TG, t = threadIdx.x % TG;
+    cpx<T> *lds = lds_all + g * C::SLICE;
+    const int y = xcd_major(blockIdx.x, gridDim.x) * C::GPW + g;
+    const bool active = y < NP;
+    const int yy = active ? y : 0;
+    const int h = d.N / 2;
+
+    // w of this row (the imaginary part of the result is the rounding of a
+    // real transform and is dropped)
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = hermitian_at(Gw, t + TG * s, yy, d);
+    wave_fft<T, E, TG, 1>(v, lds, t, tw);
+    T wr[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) wr[s] = v[s].re;
+
+    // u + i v
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int k = t + TG * s;
+        const cpx<T> a = hermitian_at(Gu, k, yy, d), b = hermitian_at(Gv, k, yy, d);
+        v[s] = mk<T>(a.re - b.im, a.im + b.re);
+    }
+    wave_sync();
+    wave_fft<T, E, TG, 1>(v, lds, t, tw);
+    T vmax = (T)0;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const T m = absv(v[s].re) + absv(v[s].im);
+        vmax = (m > vmax || m != m) ? m : vmax; // NaN sticks
+        v[s] = mk<T>(v[s].re * wr[s], v[s].im * wr[s]); // uw + i vw
+    }
+    if (forward) {
+        wave_sync();
+        wave_fft<T, E, TG, -1>(v, lds, t, tw);
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < E; ++s) lds[LdsPad<T>::idx(t + TG * s)] = v[s];
+        wave_sync();
+        if (active) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const int k = t + TG * s;
+                if (k <= h) {
+                    const cpx<T> P = lds[LdsPad<T>::idx(k)];
+                    cpx<T> Pc = lds[LdsPad<T>::idx((NP - k) % NP)];
+                    Pc.im = -Pc.im;
+                    const cpx<T> su = add(P, Pc), df = sub(P, Pc);
+                    Huw[(size_t)k * NP + y] = mk<T>((T)0.5 * su.re, (T)0.5 * su.im);
+                    Hvw[(size_t)k * NP + y] = mk<T>((T)0.5 * df.im, (T)-0.5 * df.re);
+                }
+            }
+        }
+    }
+    if (!track_max) return;
+    double dm = active ? (double)vmax : 0.0;
+    if (dm != dm) dm = __longlong_as_double(0x7ff8000000000000ll);
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(dm, off);
+        dm = (o > dm || o != o) ? o : dm;
+    }
+    if (threadIdx.x == 0) atomicMax(&sc->vmax_bits, (unsigned long long)__double_as_longlong(dm));
+}
+
+// ------------------------------------------------------------- pass C
+template <typename T, int NP>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void k2_passC(const cpx<T> *__restrict__ Huw,
+                                                const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
+                                                const cpx<T> *Wst, cpx<T> *Wout, FusedDims d,
+                                                const cpx<T> *__restrict__ tw,
+                                                const int *__restrict__ trunc_i1, DevScalars *sc,
+                                                StageArgs a)
+{
+#pragma clang fp contract(off)
+    using C = WaveCfg<T, NP>;
+    constexpr int E = C::E, TG = C::TG;
+    __shared__ cpx<T> lds_all[2 * C::GPW * C::SLICE];
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64; // wv 0: uw, 1: vw
+    const int g = lane / TG, t = lane % TG;
+    cpx<T> *mine = lds_all + (wv * C::GPW + g) * C::SLICE;
+    cpx<T> *other = lds_all + ((1 - wv) * C::GPW + g) * C::SLICE;
+    const int kx = blockIdx.x * C::GPW + g;
+    const bool active = kx < d.K;
+    const cpx<T> *src = (wv == 0 ? Huw : Hvw) + (size_t)(active ? kx : 0) * NP;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
+    wave_fft<T, E, TG, -1>(v, mine, t, tw);
+    // wave 0 keeps the even s and hands over the odd s, wave 1 the reverse
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+        if ((s & 1) != wv) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
+    __syncthreads();
+
+    const double dtd = stage_dt(a, sc);
+    const T dt = (T)dtd;
+    const T qx = (T)kx;
+    const size_t col = (size_t)(active ? kx : 0) * d.KP;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        if ((s & 1) != wv || !active) continue;
+        const int j = t + TG * s;
+        const int c = compact_row(j, d);
+        if (c < 0) continue;
+        const int kyi = (j <= NP / 2) ? j : j - NP;
+        const T qy = (T)kyi;
+        cpx<T> out = mk<T>((T)0, (T)0);
+        if (kx < trunc_i1[j]) {
+            const cpx<T> o = other[LdsPad<T>::idx(j)];
+            const cpx<T> P = div_real(wv == 0 ? v[s] : o, a.norm);
+            const cpx<T> Q = div_real(wv == 0 ? o : v[s], a.norm);
+            const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
+            const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
+            const T sn = (T)a.nu * (qx * qx + qy * qy);
+            const cpx<T> ws = Wst[col + c];
+            const cpx<T> dW = mk<T>((-du.re - dv.re) - sn * ws.re, (-du.im - dv.im) - sn * ws.im);
+            const cpx<T> w0 = Wn[col + c];
+            if (a.stage == 1) {
+                out = mk<T>(w0.re + dt * dW.re, w0.im + dt * dW.im);
+            } else if (a.stage == 2) {
+                const T cc = (T)0.25 * dt;
+                out = mk<T>(((T)0.75 * w0.re + (T)0.25 * ws.re) + cc * dW.re,
+                            ((T)0.75 * w0.im + (T)0.25 * ws.im) + cc * dW.im);
+            } else {
+                const T cc = dt * (T)2;
+                out = mk<T>((w0.re / (T)3 + ((T)2 * ws.re) / (T)3) + (cc * dW.re) / (T)3,
+                            (w0.im / (T)3 + ((T)2 * ws.im) / (T)3) + (cc * dW.im) / (T)3);
+            }
+        }
+        Wout[col + c] = out;
+    }
+    if (a.stage == 3 && blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->dt = dtd;
+        sc->time = sc->time + dtd;
+        sc->nadv = sc->nadv + 1;
+    }
+}
+
+} // namespace s2d

@@ -11,11 +11,13 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "kernels_fused.hpp"
+#include "kernels_fused_v2.hpp"
 #include "kernels_ref.hpp"
 #include "s2d_internal.hpp"
 
@@ -586,9 +588,104 @@ int fused_steps_E(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
     return S2D_OK;
 }
 
+// ---- wave-FFT kernels for the compiled padded sizes
+
+template <typename T, int NP>
+int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
+{
+    ProfScope ps(c, S2D_KERNEL_PASS_A);
+    using Cf = WaveCfg<T, NP>;
+    FusedDims d = fdims(c);
+    const int blocks = (3 * d.K + Cf::GPW - 1) / Cf::GPW;
+    hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(64), 0, c->stream, src,
+                       (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2], d,
+                       (const cpx<T> *)c->col.tw, c->d_sc, reset);
+    S2D_LAUNCHED(c, "k2_passA");
+    return S2D_OK;
+}
+
+template <typename T, int NP>
+int launch2_passB(s2d_ctx *c, int forward, int track_max)
+{
+    ProfScope ps(c, S2D_KERNEL_PASS_B);
+    using Cf = WaveCfg<T, NP>;
+    FusedDims d = fdims(c);
+    const int blocks = (NP + Cf::GPW - 1) / Cf::GPW;
+    hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(64), 0, c->stream,
+                       (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2],
+                       (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d, (const cpx<T> *)c->rowf.tw,
+                       c->d_sc, forward, track_max);
+    S2D_LAUNCHED(c, "k2_passB");
+    return S2D_OK;
+}
+
+template <typename T, int NP>
+int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &a)
+{
+    ProfScope ps(c, S2D_KERNEL_PASS_C);
+    using Cf = WaveCfg<T, NP>;
+    FusedDims d = fdims(c);
+    const int blocks = (d.K + Cf::GPW - 1) / Cf::GPW;
+    hipLaunchKernelGGL((k2_passC<T, NP>), dim3(blocks), dim3(128), 0, c->stream,
+                       (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
+                       Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a);
+    S2D_LAUNCHED(c, "k2_passC");
+    return S2D_OK;
+}
+
+template <typename T, int NP>
+int fused_steps_wave(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
+{
+    cpx<T> *Wn = (cpx<T> *)c->Wn, *Ws = (cpx<T> *)c->Ws;
+    StageArgs a;
+    a.nu = nu;
+    a.dt = dt;
+    a.cfl = cfl;
+    a.dx = 2.0 * M_PI / (double)c->N;
+    a.norm = (double)c->g.nxG * (double)c->g.nyG;
+    for (int st = 0; st < nsteps; ++st) {
+        for (int stage = 1; stage <= 3; ++stage) {
+            const cpx<T> *src = stage == 1 ? Wn : Ws;
+            cpx<T> *dst = stage == 3 ? Wn : Ws;
+            a.stage = stage;
+            S2D_TRY((launch2_passA<T, NP>(c, src, stage == 1)));
+            S2D_TRY((launch2_passB<T, NP>(c, 1, stage == 1)));
+            S2D_TRY((launch2_passC<T, NP>(c, src, dst, a)));
+        }
+    }
+    return S2D_OK;
+}
+
+// Padded sizes with compiled wave-FFT kernels (FP64 up to 1536; FP32 also
+// 3072); 0 when the generic kernels are used.
+template <typename T>
+int wave_np(const s2d_ctx *c)
+{
+    if (!c->wave_kernels) return 0;
+    const int np = c->g.nxG;
+    switch (np) {
+    case 96: case 192: case 384: case 768: case 1536: return np;
+    case 3072: return sizeof(T) == 4 ? np : 0;
+    }
+    return 0;
+}
+
+#define S2D_WAVE_SWITCH(T, np, CALL)                            \
+    switch (np) {                                               \
+    case 96: { constexpr int NP = 96; CALL; } break;            \
+    case 192: { constexpr int NP = 192; CALL; } break;          \
+    case 384: { constexpr int NP = 384; CALL; } break;          \
+    case 768: { constexpr int NP = 768; CALL; } break;          \
+    case 1536: { constexpr int NP = 1536; CALL; } break;        \
+    case 3072: if constexpr (sizeof(T) == 4) { constexpr int NP = 3072; CALL; } break; \
+    }
+
 template <typename T>
 int fused_steps(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
 {
+    if (const int np = wave_np<T>(c)) {
+        S2D_WAVE_SWITCH(T, np, return (fused_steps_wave<T, NP>(c, nsteps, nu, dt, cfl)))
+    }
     const int EC = c->col.E, ER = c->rowf.E;
 #define S2D_DISPATCH(ec, er) \
     if (EC == ec && ER == er) return fused_steps_E<T, ec, er>(c, nsteps, nu, dt, cfl);
@@ -609,7 +706,10 @@ int fused_vmax(s2d_ctx *c, double *vmax)
     if (EC != ER) return fail(c, S2D_ERR_UNSUPPORTED, "no fused kernel for E=(%d,%d)", EC, ER);
     int r = S2D_ERR_UNSUPPORTED;
     const cpx<T> *Wn = (const cpx<T> *)c->Wn;
-    switch (EC) {
+    if (const int np = wave_np<T>(c)) {
+        S2D_WAVE_SWITCH(T, np, r = (launch2_passA<T, NP>(c, Wn, 1));
+                        if (!r) r = (launch2_passB<T, NP>(c, 0, 1)))
+    } else switch (EC) {
     case 12: r = launch_passA<T, 12>(c, Wn, 1); if (!r) r = launch_passB<T, 12>(c, 0, 1); break;
     case 8: r = launch_passA<T, 8>(c, Wn, 1); if (!r) r = launch_passB<T, 8>(c, 0, 1); break;
     case 6: r = launch_passA<T, 6>(c, Wn, 1); if (!r) r = launch_passB<T, 6>(c, 0, 1); break;
@@ -777,6 +877,10 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
         return bail(S2D_ERR_HIP);
     }
     c->max_lds = (int)prop.sharedMemPerBlock;
+    {
+        const char *e = std::getenv("S2D_GENERIC_KERNELS");
+        c->wave_kernels = !(e && e[0] == '1');
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(S2D_ERR_HIP);

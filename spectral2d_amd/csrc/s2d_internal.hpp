@@ -62,6 +62,7 @@ struct s2d_ctx {
 
     // fused compact state
     bool fused = false;
+    bool wave_kernels = true; // compiled wave-FFT kernels for the hot sizes
     int N = 0, K = 0, KP = 0; // resolved size, kept kx count, compact column pitch
     void *Wn = nullptr, *Ws = nullptr;
     void *G[3] = {}, *H[2] = {};
