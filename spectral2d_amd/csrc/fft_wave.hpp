@@ -1,22 +1,23 @@
-// fft_wave.hpp -- compile-time Stockham FFT on a lane group of one wave.
+// fft_wave.hpp -- compile-time Stockham FFT on a lane group.
 //
-// A length n = E*TG transform (n = 2^a 3^b) is carried by TG lanes of one
-// wave (TG a power of two <= 64; 64/TG transforms share a wave). Lane t of
-// its group holds x[t + TG*s], s < E. The early passes are radix-R
-// Stockham passes (R | E, the radices multiply to TG) that exchange through
-// the group's LDS slice; the final pass is one radix-E butterfly per lane,
-// entirely in registers, whose outputs X[t + TG*m] are again the lane's own
-// elements. So a transform costs (number of early passes) LDS exchanges:
-// two for n = 1536 (8 x 8 x 24) or 3072 (8 x 8 x 48).
+// A length n = E*TG transform (n = 2^a 3^b) is carried by a group of TG
+// lanes (TG a power of two, 2..256). Groups of up to 64 lanes live inside
+// one wave (64/TG groups per wave); larger groups span TG/64 waves of one
+// workgroup. Lane t of the group holds x[t + TG*s], s < E. The early passes
+// are radix-R Stockham passes (R in {8, 4, 2}, R | E, the radices multiply
+// to TG) that exchange through the group's LDS slice; the final pass is one
+// radix-E butterfly per lane, in registers, whose outputs X[t + TG*m] are
+// again the lane's own elements.
 //
 // The LDS slice is padded with one slot after every 128 bytes, which keeps
-// both the strided Stockham writes and the contiguous reads free of bank
-// conflicts. Only lanes of one wave touch a slice, so ordering needs
-// wave-scope fences, not workgroup barriers.
+// the strided Stockham writes and the contiguous reads free of bank
+// conflicts. Groups inside one wave synchronise with wave-scope fences;
+// multi-wave groups use workgroup barriers, which every thread of the
+// workgroup must reach the same number of times.
 //
 // Twiddles come from the FP64-generated table tw[e] = exp(-2 pi i e / n)
-// of the transform length (rounded to T); the in-register radix-E DFT uses
-// the same table at stride n/E.
+// of the transform length (rounded to T); the in-register DFTs use the same
+// table at stride n/E, read at lane-uniform addresses.
 #pragma once
 
 #include "fft_block.hpp"
@@ -30,18 +31,30 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// elements per 128-byte LDS row
+template <int TG>
+__device__ __forceinline__ void group_sync()
+{
+    if constexpr (TG > 64)
+        __syncthreads();
+    else
+        wave_sync();
+}
+
+// LDS slot of element i: bits 0-3 XORed with bits 3-6, a permutation inside
+// each aligned block of 16 elements (n is a multiple of 16 for every size
+// used). It keeps the strided Stockham writes and the contiguous reads
+// within 1.25-1.5x of conflict-free for both 32- and 64-bank LDS, with no
+// padding.
 template <typename T>
 struct LdsPad {
-    static constexpr int shift = sizeof(T) == 8 ? 3 : 4;
-    static __device__ __forceinline__ int idx(int i) { return i + (i >> shift); }
-    static constexpr int size(int n) { return n + (n >> shift) + 1; }
+    static __device__ __forceinline__ int idx(int i) { return i ^ ((i >> 3) & 15); }
+    static constexpr int size(int n) { return n; }
 };
 
 // In-register DFT of compile-time size N on a[0..N-1] (natural order in and
 // out). Base radices 2, 3, 4, 8; composite N = N1*N2 by Cooley-Tukey with
 // N1 the largest base radix dividing N. Internal twiddles W_N^e are read
-// from the length-n table at stride n/N (uniform across lanes).
+// from the length-n table at stride n/N.
 template <typename T, int N, int D>
 __device__ __forceinline__ void dft_ct(cpx<T> *a, const cpx<T> *__restrict__ tw, int n)
 {
@@ -50,7 +63,6 @@ __device__ __forceinline__ void dft_ct(cpx<T> *a, const cpx<T> *__restrict__ tw,
     } else {
         constexpr int N1 = (N % 8 == 0) ? 8 : (N % 4 == 0) ? 4 : (N % 3 == 0) ? 3 : 2;
         constexpr int N2 = N / N1;
-        // step 1: for each n2, DFT_N1 over n1 of a[N2*n1 + n2]
         cpx<T> y[N];
 #pragma unroll
         for (int n2 = 0; n2 < N2; ++n2) {
@@ -60,12 +72,10 @@ __device__ __forceinline__ void dft_ct(cpx<T> *a, const cpx<T> *__restrict__ tw,
             dft<T, N1, D>(b);
 #pragma unroll
             for (int k1 = 0; k1 < N1; ++k1) {
-                // step 2: twiddle W_N^{n2 k1}
                 if (n2 > 0 && k1 > 0) b[k1] = mul(b[k1], twiddle<T, D>(tw, (n / N) * n2 * k1));
                 y[n2 * N1 + k1] = b[k1];
             }
         }
-        // step 3: for each k1, DFT_N2 over n2 -> X[k1 + N1*k2]
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) {
             cpx<T> c[N2];
@@ -78,21 +88,18 @@ __device__ __forceinline__ void dft_ct(cpx<T> *a, const cpx<T> *__restrict__ tw,
     }
 }
 
-// Early radices of a lane group of TG lanes (their product is TG).
-template <int TG>
-struct GroupRadices;
-template <> struct GroupRadices<2> { static constexpr int np = 1; static constexpr int r[2] = {2, 1}; };
-template <> struct GroupRadices<4> { static constexpr int np = 1; static constexpr int r[2] = {4, 1}; };
-template <> struct GroupRadices<8> { static constexpr int np = 1; static constexpr int r[2] = {8, 1}; };
-template <> struct GroupRadices<16> { static constexpr int np = 2; static constexpr int r[2] = {4, 4}; };
-template <> struct GroupRadices<32> { static constexpr int np = 2; static constexpr int r[2] = {8, 4}; };
-template <> struct GroupRadices<64> { static constexpr int np = 2; static constexpr int r[2] = {8, 8}; };
+// Largest early radix in {8, 4, 2} dividing both E and the remaining
+// product rem of the group size.
+constexpr int early_radix(int E, int rem)
+{
+    return (E % 8 == 0 && rem % 8 == 0) ? 8 : (E % 4 == 0 && rem % 4 == 0) ? 4 : 2;
+}
 
-// One early radix-R pass with Ls = product of the previous radices (which
+// One early radix-R pass with Ls = product of the previous radices (Ls
 // divides TG, so k = j % Ls = t % Ls for every butterfly of the lane).
 template <typename T, int E, int TG, int R, int Ls, int D>
-__device__ __forceinline__ void wave_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, int n,
-                                          const cpx<T> *__restrict__ tw)
+__device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, int n,
+                                           const cpx<T> *__restrict__ tw)
 {
     constexpr int Q = E / R;
     const int k = t & (Ls - 1);
@@ -119,29 +126,39 @@ __device__ __forceinline__ void wave_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, in
     }
 }
 
-template <typename T, int E, int TG>
-__device__ __forceinline__ void wave_gather(cpx<T> (&v)[E], const cpx<T> *lds, int t)
+template <typename T, int E, int TG, int D, int Ls>
+__device__ __forceinline__ void early_passes(cpx<T> (&v)[E], cpx<T> *lds, int t,
+                                             const cpx<T> *__restrict__ tw)
 {
+    if constexpr (Ls < TG) {
+        constexpr int R = early_radix(E, TG / Ls);
+        static_assert(E % R == 0 && (TG / Ls) % R == 0, "no early radix");
+        group_pass<T, E, TG, R, Ls, D>(v, lds, t, E * TG, tw);
+        group_sync<TG>();
 #pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = lds[LdsPad<T>::idx(t + TG * s)];
+        for (int s = 0; s < E; ++s) v[s] = lds[LdsPad<T>::idx(t + TG * s)];
+        group_sync<TG>();
+        early_passes<T, E, TG, D, Ls * R>(v, lds, t, tw);
+    }
 }
 
 // Final radix-E pass of lane t (Ls = TG, k = t): X[t + TG*k] = sum_m
-// v[m] W_n^{m t} W_E^{m k}. With E = 8*N2 and m = N2*n1 + n2 the twiddle
-// factors as W_n^{N2 n1 t} * W_n^{n2 t}: the first goes into a radix-8 DFT
-// over n1, the second (constant in n1) is applied after it together with
-// the Cooley-Tukey twiddle W_E^{n2 k1}, which is uniform across lanes.
-// That needs 7 + (N2 - 1) per-lane twiddles instead of E - 1.
+// v[m] W_n^{m t} W_E^{m k}. With E = N1*N2 (N1 in {8, 4}) and m = N2*n1 +
+// n2 the twiddle factors as W_n^{N2 n1 t} * W_n^{n2 t}: the first goes into
+// a radix-N1 DFT over n1, the second (constant in n1) is applied after it
+// together with the Cooley-Tukey twiddle W_E^{n2 k1}, which is uniform
+// across lanes. That needs N1 - 1 + N2 - 1 per-lane twiddles, not E - 1.
 template <typename T, int E, int TG, int D>
 __device__ __forceinline__ void final_pass(cpx<T> (&v)[E], int t, const cpx<T> *__restrict__ tw)
 {
     constexpr int n = E * TG;
-    if constexpr (E % 8 != 0) {
+    constexpr int N1 = (E % 8 == 0) ? 8 : (E % 4 == 0) ? 4 : 0;
+    if constexpr (N1 == 0 || E == N1) {
 #pragma unroll
         for (int m = 1; m < E; ++m) v[m] = mul(v[m], twiddle<T, D>(tw, m * t));
         dft_ct<T, E, D>(v, tw, n);
     } else {
-        constexpr int N1 = 8, N2 = E / 8;
+        constexpr int N2 = E / N1;
         cpx<T> r[N1];
 #pragma unroll
         for (int n1 = 1; n1 < N1; ++n1) r[n1] = twiddle<T, D>(tw, N2 * n1 * t);
@@ -179,22 +196,10 @@ __device__ __forceinline__ void final_pass(cpx<T> (&v)[E], int t, const cpx<T> *
 // Full transform of the group: on entry v[s] = x[t + TG*s], on exit
 // v[s] = X[t + TG*s]. lds is the group's slice (LdsPad<T>::size(n)).
 template <typename T, int E, int TG, int D>
-__device__ __forceinline__ void wave_fft(cpx<T> (&v)[E], cpx<T> *lds, int t,
-                                         const cpx<T> *__restrict__ tw)
+__device__ __forceinline__ void group_fft(cpx<T> (&v)[E], cpx<T> *lds, int t,
+                                          const cpx<T> *__restrict__ tw)
 {
-    constexpr int n = E * TG;
-    using GR = GroupRadices<TG>;
-    static_assert(E % GR::r[0] == 0 && (GR::np < 2 || E % GR::r[1] == 0), "radix must divide E");
-    wave_pass<T, E, TG, GR::r[0], 1, D>(v, lds, t, n, tw);
-    wave_sync();
-    wave_gather<T, E, TG>(v, lds, t);
-    wave_sync();
-    if constexpr (GR::np == 2) {
-        wave_pass<T, E, TG, GR::r[1], GR::r[0], D>(v, lds, t, n, tw);
-        wave_sync();
-        wave_gather<T, E, TG>(v, lds, t);
-        wave_sync();
-    }
+    early_passes<T, E, TG, D, 1>(v, lds, t, tw);
     final_pass<T, E, TG, D>(v, t, tw);
 }
 

@@ -8,7 +8,8 @@
 // One RK stage = three kernels over the 3/2-padded grid (Np = 3N/2):
 //  A  (per kx column)  omega -> u_hat = i ky psi, v_hat = -i kx psi,
 //     w_hat; zero-pad to Np in FFTW ky order; inverse FFT along y; write the
-//     mixed (kx, y) fields G[f][kx*Np + y].
+//     mixed (kx, y) fields G[f][y*KG + kx] (row-major, so that pass B reads
+//     whole rows).
 //  B  (per pair of rows) two rows' real transforms share one complex
 //     transform: z3 = w_a + i w_b from the G_w spectra (Hermitian
 //     extension), then for each row z1 = u + i v, the products z1*w =
@@ -37,6 +38,7 @@ struct FusedDims {
     int K;   // kept kx columns, N/2 + 1
     int KP;  // compact column pitch
     int cI;  // reference complex row length (Np/2 + 1)
+    int KG;  // row pitch of the G fields (K rounded up to 16)
 };
 
 template <typename T>
@@ -96,9 +98,9 @@ __global__ void k_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu
         }
     }
     block_fft<T, E, 1>(v, lds + (size_t)f * d.Np, p, t, true, tw);
-    cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + (size_t)kx * d.Np;
+    cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + kx;
 #pragma unroll
-    for (int s = 0; s < E; ++s) dst[t + p.T * s] = v[s];
+    for (int s = 0; s < E; ++s) dst[(size_t)(t + p.T * s) * d.KG] = v[s];
 }
 
 // ------------------------------------------------------------- pass B
@@ -114,16 +116,16 @@ __device__ __forceinline__ cpx<T> packed_spectrum(const cpx<T> *__restrict__ GA,
     const int h = d.N / 2;
     cpx<T> A, B;
     if (k <= h) {
-        A = GA[(size_t)k * d.Np + ya];
-        B = GB[(size_t)k * d.Np + yb];
+        A = GA[(size_t)ya * d.KG + k];
+        B = GB[(size_t)yb * d.KG + k];
         if (k == 0) {
             A.im = (T)0;
             B.im = (T)0;
         }
     } else if (k >= d.Np - h) {
         const int m = d.Np - k;
-        A = GA[(size_t)m * d.Np + ya];
-        B = GB[(size_t)m * d.Np + yb];
+        A = GA[(size_t)ya * d.KG + m];
+        B = GB[(size_t)yb * d.KG + m];
         A.im = -A.im;
         B.im = -B.im;
     } else {

@@ -1,19 +1,23 @@
-// kernels_fused_v2.hpp -- the fused RK3 stage kernels on wave-level FFTs,
-// compiled for fixed padded sizes Np = 24 * TG (TG = 4..64 lanes per
-// transform) and, in FP32, Np = 3072 = 48 * 64. Same mathematics and data
-// layout as kernels_fused.hpp (compact omega, mixed (kx, y) fields G/H);
-// other sizes use those kernels.
+// kernels_fused_v2.hpp -- the fused RK3 stage kernels on lane-group FFTs
+// (fft_wave.hpp), compiled for the padded sizes Np in {96, 192, 384, 768,
+// 1536, 3072}. Same mathematics and data layout as kernels_fused.hpp
+// (compact omega, mixed (kx, y) fields G/H); other sizes use those kernels.
 //
-//  A  one lane group per (kx, field): u_hat, v_hat or w_hat of a column,
-//     inverse transform along y.
-//  B  one lane group per row y: w from its Hermitian spectrum, u + i v
+// Elements per lane: 12 in FP64 (a 1536-point transform spans two waves,
+// four LDS exchanges, about half the registers and LDS per wave of 24),
+// 24 in FP32 (one wave, two exchanges).
+//
+//  A  one lane group per (field, kx): u_hat, v_hat or w_hat of a column,
+//     inverse transform along y, stored into the rows of G (G[y*KG + kx]).
+//  B  one lane group per row y (contiguous loads of its G rows): w from its
+//     Hermitian spectrum, u + i v
 //     (two-for-one), the products (u w + i v w) in registers, one forward
 //     transform, the uw / vw spectra separated from P_k and conj P_{Np-k}.
 //     Rows are assigned XCD-major so that the rows in flight on one XCD are
 //     adjacent and share the 128-byte lines of the (kx, y) fields in its L2.
-//  C  two waves per column group: one transforms uw, the other vw; they swap
-//     halves of their results through LDS and finish the RHS, truncation and
-//     Shu-Osher stage update on the compact state.
+//  C  two halves per column group: one transforms uw, the other vw; they
+//     swap halves of their results through LDS and finish the RHS,
+//     truncation and Shu-Osher stage update on the compact state.
 #pragma once
 
 #include "fft_wave.hpp"
@@ -23,15 +27,15 @@ namespace s2d {
 
 template <typename T, int NP>
 struct WaveCfg {
-    static constexpr int E = (NP % 48 == 0 && NP / 48 == 64) ? 48 : 24;
+    static constexpr int E = sizeof(T) == 8 ? 12 : 24;
     static constexpr int TG = NP / E;
-    static constexpr int GPW = 64 / TG; // transforms per wave
+    static constexpr int GPB = TG >= 64 ? 1 : 64 / TG; // groups per block (A, B)
+    static constexpr int BLK = TG >= 64 ? TG : 64;      // threads per block (A, B)
     static constexpr int SLICE = LdsPad<T>::size(NP);
-    // register budget: at least this many waves per SIMD
-    static constexpr int WPE = (sizeof(T) == 8 || E == 48) ? 2 : 3;
-    static constexpr int WPE_B = (sizeof(T) == 8 || E == 48) ? 1 : 2; // pass B holds w and u + i v
-    static_assert(E * TG == NP && TG >= 2 && TG <= 64 && (TG & (TG - 1)) == 0,
-                  "unsupported wave FFT size");
+    static constexpr int WPE = sizeof(T) == 8 ? 3 : 2; // register budget, waves per SIMD
+    static constexpr int WPE_B = 2;                    // pass B holds w and u + i v
+    static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
+                  "unsupported group FFT size");
 };
 
 // XCD-major block order: the hardware deals blocks round-robin to 8 XCDs,
@@ -45,20 +49,22 @@ __device__ __forceinline__ int xcd_major(int b, int nblocks)
 
 // ------------------------------------------------------------- pass A
 template <typename T, int NP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void k2_passA(const cpx<T> *__restrict__ Wsrc,
-                                               cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
-                                               cpx<T> *__restrict__ Gw, FusedDims d,
-                                               const cpx<T> *__restrict__ tw, DevScalars *sc,
-                                               int reset_max)
+__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
+k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
+         cpx<T> *__restrict__ Gw, FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc,
+         int reset_max)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     constexpr int E = C::E, TG = C::TG;
-    __shared__ cpx<T> lds[C::GPW * C::SLICE];
+    __shared__ cpx<T> lds[C::GPB * C::SLICE];
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
-    const int job = blockIdx.x * C::GPW + g;
+    // field-major jobs, XCD-major blocks: the blocks in flight on one XCD
+    // hold adjacent kx of one field and fill whole lines of the G rows
+    const int job = xcd_major(blockIdx.x, gridDim.x) * C::GPB + g;
     const bool active = job < 3 * d.K;
-    const int kx = active ? job / 3 : 0, f = job % 3;
+    const int f = active ? job / d.K : 0, kx = active ? job % d.K : 0;
     if (reset_max && blockIdx.x == 0 && threadIdx.x == 0) {
         sc->vmax_bits = 0ull;
         sc->bad = 0u;
@@ -77,9 +83,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, N
             v[s] = w;
         } else {
             cpx<T> ps = mk<T>((T)0, (T)0);
-            if (!(kx == 0 && kyi == 0)) {
+            if (!(kx == 0 && kyi == 0) && c >= 0) {
                 const T dd = (kx == 0) ? qy * qy : (kyi == 0) ? qx * qx : qx * qx + qy * qy;
-                ps = mk<T>(w.re / dd, w.im / dd);
+                const T inv = (T)1 / dd; // one reciprocal per mode
+                ps = mk<T>(w.re * inv, w.im * inv);
             }
             if (f == 0) {
                 v[s] = mk<T>(-(qy * ps.im), qy * ps.re); // II*ky*psi
@@ -89,11 +96,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, N
             }
         }
     }
-    wave_fft<T, E, TG, 1>(v, lds + g * C::SLICE, t, tw);
+    group_fft<T, E, TG, 1>(v, lds + g * C::SLICE, t, tw);
     if (active) {
-        cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + (size_t)kx * NP;
+        cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + kx;
 #pragma unroll
-        for (int s = 0; s < E; ++s) dst[t + TG * s] = v[s];
+        for (int s = 0; s < E; ++s) dst[(size_t)(t + TG * s) * d.KG] = v[s];
     }
 }
 
@@ -107,12 +114,12 @@ __device__ __forceinline__ cpx<T> hermitian_at(const cpx<T> *__restrict__ G, int
 {
     const int h = d.N / 2;
     if (k <= h) {
-        cpx<T> a = G[(size_t)k * d.Np + y];
+        cpx<T> a = G[(size_t)y * d.KG + k];
         if (k == 0) a.im = (T)0;
         return a;
     }
     if (k >= d.Np - h) {
-        cpx<T> a = G[(size_t)(d.Np - k) * d.Np + y];
+        cpx<T> a = G[(size_t)y * d.KG + (d.Np - k)];
         a.im = -a.im;
         return a;
     }
@@ -120,20 +127,19 @@ __device__ __forceinline__ cpx<T> hermitian_at(const cpx<T> *__restrict__ G, int
 }
 
 template <typename T, int NP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void k2_passB(const cpx<T> *__restrict__ Gu,
-                                               const cpx<T> *__restrict__ Gv,
-                                               const cpx<T> *__restrict__ Gw,
-                                               cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
-                                               FusedDims d, const cpx<T> *__restrict__ tw,
-                                               DevScalars *sc, int forward, int track_max)
+__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void
+k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
+         const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
+         FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc, int forward, int track_max)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     constexpr int E = C::E, TG = C::TG;
-    __shared__ cpx<T> lds_all[C::GPW * C::SLICE];
+    __shared__ cpx<T> lds_all[C::GPB * C::SLICE];
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
     cpx<T> *lds = lds_all + g * C::SLICE;
-    const int y = xcd_major(blockIdx.x, gridDim.x) * C::GPW + g;
+    const int y = xcd_major(blockIdx.x, gridDim.x) * C::GPB + g;
     const bool active = y < NP;
     const int yy = active ? y : 0;
     const int h = d.N / 2;
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, N
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = hermitian_at(Gw, t + TG * s, yy, d);
-    wave_fft<T, E, TG, 1>(v, lds, t, tw);
+    group_fft<T, E, TG, 1>(v, lds, t, tw);
     T wr[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) wr[s] = v[s].re;
@@ -155,8 +161,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, N
         const cpx<T> a = hermitian_at(Gu, k, yy, d), b = hermitian_at(Gv, k, yy, d);
         v[s] = mk<T>(a.re - b.im, a.im + b.re);
     }
-    wave_sync();
-    wave_fft<T, E, TG, 1>(v, lds, t, tw);
+    group_fft<T, E, TG, 1>(v, lds, t, tw);
     T vmax = (T)0;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
@@ -165,12 +170,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, N
         v[s] = mk<T>(v[s].re * wr[s], v[s].im * wr[s]); // uw + i vw
     }
     if (forward) {
-        wave_sync();
-        wave_fft<T, E, TG, -1>(v, lds, t, tw);
-        wave_sync();
+        group_fft<T, E, TG, -1>(v, lds, t, tw);
 #pragma unroll
         for (int s = 0; s < E; ++s) lds[LdsPad<T>::idx(t + TG * s)] = v[s];
-        wave_sync();
+        group_sync<TG>();
         if (active) {
 #pragma unroll
             for (int s = 0; s < E; ++s) {
@@ -193,46 +196,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, N
         double o = __shfl_xor(dm, off);
         dm = (o > dm || o != o) ? o : dm;
     }
-    if (threadIdx.x == 0) atomicMax(&sc->vmax_bits, (unsigned long long)__double_as_longlong(dm));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(&sc->vmax_bits, (unsigned long long)__double_as_longlong(dm));
 }
 
 // ------------------------------------------------------------- pass C
 template <typename T, int NP>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void k2_passC(const cpx<T> *__restrict__ Huw,
-                                                const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
-                                                const cpx<T> *Wst, cpx<T> *Wout, FusedDims d,
-                                                const cpx<T> *__restrict__ tw,
-                                                const int *__restrict__ trunc_i1, DevScalars *sc,
-                                                StageArgs a)
+__global__ __launch_bounds__((2 * WaveCfg<T, NP>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
+k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
+         const cpx<T> *Wst, cpx<T> *Wout, FusedDims d, const cpx<T> *__restrict__ tw,
+         const int *__restrict__ trunc_i1, DevScalars *sc, StageArgs a)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     constexpr int E = C::E, TG = C::TG;
-    __shared__ cpx<T> lds_all[2 * C::GPW * C::SLICE];
-    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64; // wv 0: uw, 1: vw
+    __shared__ cpx<T> lds_all[2 * C::GPB * C::SLICE];
+    const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK; // half 0: uw, 1: vw
     const int g = lane / TG, t = lane % TG;
-    cpx<T> *mine = lds_all + (wv * C::GPW + g) * C::SLICE;
-    cpx<T> *other = lds_all + ((1 - wv) * C::GPW + g) * C::SLICE;
-    const int kx = blockIdx.x * C::GPW + g;
+    cpx<T> *mine = lds_all + (half * C::GPB + g) * C::SLICE;
+    cpx<T> *other = lds_all + ((1 - half) * C::GPB + g) * C::SLICE;
+    const int kx = blockIdx.x * C::GPB + g;
     const bool active = kx < d.K;
-    const cpx<T> *src = (wv == 0 ? Huw : Hvw) + (size_t)(active ? kx : 0) * NP;
+    const cpx<T> *src = (half == 0 ? Huw : Hvw) + (size_t)(active ? kx : 0) * NP;
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
-    wave_fft<T, E, TG, -1>(v, mine, t, tw);
-    // wave 0 keeps the even s and hands over the odd s, wave 1 the reverse
+    group_fft<T, E, TG, -1>(v, mine, t, tw);
+    // half 0 keeps the even s and hands over the odd s, half 1 the reverse
 #pragma unroll
     for (int s = 0; s < E; ++s)
-        if ((s & 1) != wv) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
+        if ((s & 1) != half) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
     __syncthreads();
 
     const double dtd = stage_dt(a, sc);
     const T dt = (T)dtd;
     const T qx = (T)kx;
+    const T inv_norm = (T)(1.0 / a.norm); // multiplications instead of the
+                                          // reference's divisions: <= 1 ulp
     const size_t col = (size_t)(active ? kx : 0) * d.KP;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        if ((s & 1) != wv || !active) continue;
+        if ((s & 1) != half || !active) continue;
         const int j = t + TG * s;
         const int c = compact_row(j, d);
         if (c < 0) continue;
@@ -241,8 +246,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, 
         cpx<T> out = mk<T>((T)0, (T)0);
         if (kx < trunc_i1[j]) {
             const cpx<T> o = other[LdsPad<T>::idx(j)];
-            const cpx<T> P = div_real(wv == 0 ? v[s] : o, a.norm);
-            const cpx<T> Q = div_real(wv == 0 ? o : v[s], a.norm);
+            const cpx<T> Pr = half == 0 ? v[s] : o, Qr = half == 0 ? o : v[s];
+            const cpx<T> P = mk<T>(Pr.re * inv_norm, Pr.im * inv_norm);
+            const cpx<T> Q = mk<T>(Qr.re * inv_norm, Qr.im * inv_norm);
             const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
             const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
             const T sn = (T)a.nu * (qx * qx + qy * qy);
@@ -257,8 +263,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(WaveCfg<T, 
                             ((T)0.75 * w0.im + (T)0.25 * ws.im) + cc * dW.im);
             } else {
                 const T cc = dt * (T)2;
-                out = mk<T>((w0.re / (T)3 + ((T)2 * ws.re) / (T)3) + (cc * dW.re) / (T)3,
-                            (w0.im / (T)3 + ((T)2 * ws.im) / (T)3) + (cc * dW.im) / (T)3);
+                // (w + 2 w2 + 2 dt dW)/3 with one correctly rounded division:
+                // a rounded 1/3 multiplier would bias |w| by its rounding
+                // error every step
+                out = mk<T>(((w0.re + (T)2 * ws.re) + cc * dW.re) / (T)3,
+                            ((w0.im + (T)2 * ws.im) + cc * dW.im) / (T)3);
             }
         }
         Wout[col + c] = out;

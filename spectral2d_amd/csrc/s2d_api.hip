@@ -458,6 +458,7 @@ FusedDims fdims(const s2d_ctx *c)
     d.K = c->K;
     d.KP = c->KP;
     d.cI = c->g.cISize;
+    d.KG = c->KG;
     return d;
 }
 
@@ -596,8 +597,9 @@ int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
     ProfScope ps(c, S2D_KERNEL_PASS_A);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    const int blocks = (3 * d.K + Cf::GPW - 1) / Cf::GPW;
-    hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(64), 0, c->stream, src,
+    int blocks = (3 * d.K + Cf::GPB - 1) / Cf::GPB;
+    blocks = (blocks + 7) / 8 * 8; // a multiple of 8 for the XCD-major order
+    hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(Cf::BLK), 0, c->stream, src,
                        (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2], d,
                        (const cpx<T> *)c->col.tw, c->d_sc, reset);
     S2D_LAUNCHED(c, "k2_passA");
@@ -610,8 +612,8 @@ int launch2_passB(s2d_ctx *c, int forward, int track_max)
     ProfScope ps(c, S2D_KERNEL_PASS_B);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    const int blocks = (NP + Cf::GPW - 1) / Cf::GPW;
-    hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(64), 0, c->stream,
+    const int blocks = (NP + Cf::GPB - 1) / Cf::GPB;
+    hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(Cf::BLK), 0, c->stream,
                        (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2],
                        (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d, (const cpx<T> *)c->rowf.tw,
                        c->d_sc, forward, track_max);
@@ -625,8 +627,8 @@ int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &
     ProfScope ps(c, S2D_KERNEL_PASS_C);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    const int blocks = (d.K + Cf::GPW - 1) / Cf::GPW;
-    hipLaunchKernelGGL((k2_passC<T, NP>), dim3(blocks), dim3(128), 0, c->stream,
+    const int blocks = (d.K + Cf::GPB - 1) / Cf::GPB;
+    hipLaunchKernelGGL((k2_passC<T, NP>), dim3(blocks), dim3(2 * Cf::BLK), 0, c->stream,
                        (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
                        Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a);
     S2D_LAUNCHED(c, "k2_passC");
@@ -656,16 +658,15 @@ int fused_steps_wave(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
     return S2D_OK;
 }
 
-// Padded sizes with compiled wave-FFT kernels (FP64 up to 1536; FP32 also
-// 3072); 0 when the generic kernels are used.
+// Padded sizes with compiled lane-group FFT kernels; 0 when the generic
+// kernels are used.
 template <typename T>
 int wave_np(const s2d_ctx *c)
 {
     if (!c->wave_kernels) return 0;
     const int np = c->g.nxG;
     switch (np) {
-    case 96: case 192: case 384: case 768: case 1536: return np;
-    case 3072: return sizeof(T) == 4 ? np : 0;
+    case 96: case 192: case 384: case 768: case 1536: case 3072: return np;
     }
     return 0;
 }
@@ -677,7 +678,7 @@ int wave_np(const s2d_ctx *c)
     case 384: { constexpr int NP = 384; CALL; } break;          \
     case 768: { constexpr int NP = 768; CALL; } break;          \
     case 1536: { constexpr int NP = 1536; CALL; } break;        \
-    case 3072: if constexpr (sizeof(T) == 4) { constexpr int NP = 3072; CALL; } break; \
+    case 3072: { constexpr int NP = 3072; CALL; } break;        \
     }
 
 template <typename T>
@@ -957,8 +958,9 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
         c->N = nx;
         c->K = nx / 2 + 1;
         c->KP = ((nx + 1) + 7) / 8 * 8;
+        c->KG = (c->K + 15) / 16 * 16;
         const size_t ncomp = (size_t)c->K * c->KP * c->celt;
-        const size_t nmix = (size_t)c->K * g.nxG * c->celt;
+        const size_t nmix = (size_t)c->KG * g.nxG * c->celt; // G: Np rows of KG; H: K columns of Np
         if (hipMalloc(&c->Wn, ncomp) || hipMalloc(&c->Ws, ncomp)) {
             c->err = "device allocation of the compact state failed";
             return bail(S2D_ERR_ALLOC);

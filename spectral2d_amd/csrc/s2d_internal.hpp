@@ -64,6 +64,7 @@ struct s2d_ctx {
     bool fused = false;
     bool wave_kernels = true; // compiled wave-FFT kernels for the hot sizes
     int N = 0, K = 0, KP = 0; // resolved size, kept kx count, compact column pitch
+    int KG = 0;               // row pitch of the mixed G fields
     void *Wn = nullptr, *Ws = nullptr;
     void *G[3] = {}, *H[2] = {};
     s2d::DevScalars *d_sc = nullptr;

@@ -198,6 +198,70 @@ def test_fused_cfl_mode(N, steps):
     s.close()
 
 
+@pytest.mark.parametrize("N,steps", [(64, 50), (96, 30), (256, 40), (512, 20)])
+def test_fp32_error_vs_fp64_oracle(N, steps):
+    """FP32 transforms, products and updates (FP64 twiddles, FP64 CFL and
+    diagnostics): relative L2 error against the FP64 oracle, reported."""
+    from spectral2d_amd.spectral import FP32
+    print(f"[parity] FP32 N={N} steps={steps}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = _turbulence(o, 6)
+    dt = o.cfl_dt(psi, 0.5)
+    st = o.new_state()
+    st["w"][:] = w
+    st["psi"][:] = psi
+    for _ in range(steps):
+        o.step(st, 1e-3, dt)
+    s = Spectral2D(N, precision=FP32)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.rk3_steps(steps, 1e-3, dt)
+    e = rel_l2(s.cview(s.get_field("w")), o.cview(st["w"]))
+    print(f"  FP32 N={N} steps={steps} rel_l2(w) vs FP64 oracle = {e:.3e}")
+    assert e < 2e-5
+    # FP64 diagnostics of the FP32 state: TKE agrees to FP32 rounding
+    u, v = o.field(), o.field()
+    o.compute_velocity(u, v, st["psi"])
+    tke_ref = o.spectrum(u, v)[2]
+    assert abs(s.diagnostics()["tke"] - tke_ref) < 1e-5 * tke_ref
+    s.close()
+
+
+@pytest.mark.parametrize("N,steps", [(512, 10), (2048, 3)])
+def test_fused_large_sizes(N, steps):
+    """The compiled 768- and 3072-point kernels against the oracle."""
+    import os
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    o, s, st = _run_pair(N, steps, 1e-4, fused=True)
+    e = rel_l2(s.cview(s.get_field("w")), o.cview(st["w"]))
+    print(f"  fused N={N} steps={steps} rel_l2(w)={e:.3e}")
+    assert e < 1e-12
+    s.close()
+
+
+def test_generic_and_compiled_kernels_agree():
+    """S2D_GENERIC_KERNELS=1 selects the runtime-plan kernels for a size that
+    has compiled kernels; both must give the same trajectory to rounding."""
+    import os
+    N = 256
+    print(f"[parity] generic vs compiled kernels N={N}", flush=True)
+    o = oracle.Oracle(N)
+    w, psi = _turbulence(o, 9)
+    out = []
+    for flag in ("1", "0"):
+        os.environ["S2D_GENERIC_KERNELS"] = flag
+        try:
+            s = Spectral2D(N)
+        finally:
+            os.environ.pop("S2D_GENERIC_KERNELS", None)
+        s.set_field("w", w)
+        s.set_field("psi", psi)
+        s.rk3_steps(20, 1e-3, 0.0, 0.5)
+        out.append(s.cview(s.get_field("w")).copy())
+        s.close()
+    assert rel_l2(out[1], out[0]) < 1e-13
+
+
 def test_nonfinite_state_is_reported():
     from spectral2d_amd import S2DError
     N = 16
