@@ -202,6 +202,44 @@ int s2d_set_taylor_green(s2d_ctx *ctx);
 int s2d_profile_begin(s2d_ctx *ctx, int max_launches);
 int s2d_profile_end(s2d_ctx *ctx, double *ms, long long *count);
 
+/* ---- ensemble of independent runs ----
+ * No reference counterpart: the reference runs one simulation per process
+ * group (src/Spectral2D.F90). An ensemble holds nruns decaying-turbulence
+ * runs of the 3/2-rule fused step, run r on an n[r] x n[r] grid with its
+ * own viscosity, CFL time step, simulation time and step count. Runs of
+ * one size are batched; one launch per kernel pass advances every run of
+ * every size (9 launches per RK3 step). Each run computes exactly what a
+ * context of its size computes alone (s2d_create + s2d_set_turbulence +
+ * s2d_rk3_steps with the same arguments). Supported N: 64, 128, 256, 512. */
+typedef struct s2d_ensemble s2d_ensemble;
+#define S2D_ENS_GRAPH 1         /* capture one step into a hipGraph and replay it */
+const char *s2d_ensemble_last_error(const s2d_ensemble *ens);
+int s2d_ensemble_create(int nruns, const int *n, int precision, s2d_ensemble **out);
+int s2d_ensemble_destroy(s2d_ensemble *ens);
+int s2d_ensemble_size(const s2d_ensemble *ens, int *nruns, int *nclasses);
+int s2d_ensemble_run_grid(s2d_ensemble *ens, int run, s2d_grid *g);
+/* nu[nruns] */
+int s2d_ensemble_set_viscosity(s2d_ensemble *ens, const double *nu);
+/* s2d_set_turbulence of every run: seed[nruns], k0[nruns], ke[nruns]. */
+int s2d_ensemble_set_turbulence(s2d_ensemble *ens, const unsigned long long *seed,
+                                const double *k0, const double *ke);
+/* Vorticity of one run from a reference-layout host buffer (as
+ * s2d_set_field_host for S2D_FIELD_W). */
+int s2d_ensemble_set_vorticity(s2d_ensemble *ens, int run, const double *w);
+/* S2D_FIELD_W or S2D_FIELD_PSI of one run into a reference-layout host
+ * buffer (as s2d_get_field_host). */
+int s2d_ensemble_get_field(s2d_ensemble *ens, int run, int field, double *buf);
+/* nsteps RK3 steps of every run; dt and cfl as in s2d_rk3_steps (the CFL
+ * step is per run). flags: 0 or S2D_ENS_GRAPH. Every run is checked for
+ * NaN/Inf at the end. */
+int s2d_ensemble_steps(s2d_ensemble *ens, int nsteps, double dt, double cfl, int flags);
+/* out[nruns]: s2d_diagnostics of each run. */
+int s2d_ensemble_diagnostics(s2d_ensemble *ens, s2d_diag *out);
+/* Kernel timing of the ensemble launches (eager steps only), as
+ * s2d_profile_begin / s2d_profile_end. */
+int s2d_ensemble_profile_begin(s2d_ensemble *ens, int max_launches);
+int s2d_ensemble_profile_end(s2d_ensemble *ens, double *ms, long long *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,7 +44,14 @@ EXPORTS = (
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
     "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_profile_begin",
     "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
+    "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
+    "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
+    "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
+    "s2d_ensemble_steps", "s2d_ensemble_diagnostics", "s2d_ensemble_profile_begin",
+    "s2d_ensemble_profile_end",
 )
+
+ENS_GRAPH = 1
 
 
 class Grid(C.Structure):
@@ -122,8 +129,22 @@ def lib():
     L.s2d_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
     L.s2d_set_turbulence.argtypes = [vp, C.c_ulonglong, C.c_double, C.c_double]
     L.s2d_set_taylor_green.argtypes = [vp]
+    L.s2d_ensemble_last_error.argtypes = [vp]
+    L.s2d_ensemble_last_error.restype = C.c_char_p
+    L.s2d_ensemble_create.argtypes = [C.c_int, ip, C.c_int, P(vp)]
+    L.s2d_ensemble_destroy.argtypes = [vp]
+    L.s2d_ensemble_size.argtypes = [vp, ip, ip]
+    L.s2d_ensemble_run_grid.argtypes = [vp, C.c_int, P(Grid)]
+    L.s2d_ensemble_set_viscosity.argtypes = [vp, dp]
+    L.s2d_ensemble_set_turbulence.argtypes = [vp, P(C.c_ulonglong), dp, dp]
+    L.s2d_ensemble_set_vorticity.argtypes = [vp, C.c_int, dp]
+    L.s2d_ensemble_get_field.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.s2d_ensemble_steps.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int]
+    L.s2d_ensemble_diagnostics.argtypes = [vp, P(Diag)]
+    L.s2d_ensemble_profile_begin.argtypes = [vp, C.c_int]
+    L.s2d_ensemble_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
     for name in EXPORTS:
-        if name != "s2d_last_error":
+        if name not in ("s2d_last_error", "s2d_ensemble_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -132,4 +153,10 @@ def lib():
 def check(rc: int, ctx=None):
     if rc != S2D_OK:
         msg = lib().s2d_last_error(ctx)
+        raise S2DError(rc, msg.decode() if msg else "")
+
+
+def check_ens(rc: int, ens=None):
+    if rc != S2D_OK:
+        msg = lib().s2d_ensemble_last_error(ens)
         raise S2DError(rc, msg.decode() if msg else "")

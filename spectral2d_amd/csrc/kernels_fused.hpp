@@ -300,15 +300,15 @@ __global__ void k_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict
 
 // Flags a non-finite value anywhere in a complex array (sc->bad).
 template <typename T>
-__global__ void k_check_finite(const cpx<T> *__restrict__ a, size_t n, DevScalars *sc)
+__global__ void k_check_finite(const cpx<T> *__restrict__ a, size_t n, long long per_run,
+                               DevScalars *sc)
 {
-    bool ok = true;
     S2D_GRID_LOOP(k, n)
     {
         const cpx<T> x = a[k];
-        ok = ok && (x.re - x.re == (T)0) && (x.im - x.im == (T)0);
+        if (!((x.re - x.re == (T)0) && (x.im - x.im == (T)0)))
+            atomicOr(&sc[(long long)k / per_run].bad, 1u);
     }
-    if (!ok) atomicOr(&sc->bad, 1u);
 }
 
 // ---------------------------------------------------- pack / unpack

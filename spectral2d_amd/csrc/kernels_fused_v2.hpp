@@ -3,9 +3,10 @@
 // 1536, 3072}. Same mathematics and data layout as kernels_fused.hpp
 // (compact omega, mixed (kx, y) fields G/H); other sizes use those kernels.
 //
-// Elements per lane: 12 in FP64 (a 1536-point transform spans two waves,
-// four LDS exchanges, about half the registers and LDS per wave of 24),
-// 24 in FP32 (one wave, two exchanges).
+// Elements per lane: 12 in both precisions (a 1536-point transform spans
+// two waves with four LDS exchanges; 24 per lane would fit it in one wave
+// with two exchanges, but at about twice the registers per wave, which
+// spills pass B and halves the waves in flight).
 //
 //  A  one lane group per (field, kx): u_hat, v_hat or w_hat of a column,
 //     inverse transform along y, stored into the rows of G (G[y*KG + kx]).
@@ -27,13 +28,14 @@ namespace s2d {
 
 template <typename T, int NP>
 struct WaveCfg {
-    static constexpr int E = sizeof(T) == 8 ? 12 : 24;
+    static constexpr int E = 12;
     static constexpr int TG = NP / E;
     static constexpr int GPB = TG >= 64 ? 1 : 64 / TG; // groups per block (A, B)
     static constexpr int BLK = TG >= 64 ? TG : 64;      // threads per block (A, B)
     static constexpr int SLICE = LdsPad<T>::size(NP);
-    static constexpr int WPE = sizeof(T) == 8 ? 3 : 2; // register budget, waves per SIMD
-    static constexpr int WPE_B = 2;                    // pass B holds w and u + i v
+    static constexpr int WPE = sizeof(T) == 8 ? 3 : 4; // register budget, waves per SIMD
+    // pass B holds w and u + i v; a multi-wave group needs more registers
+    static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
                   "unsupported group FFT size");
 };
@@ -47,29 +49,44 @@ __device__ __forceinline__ int xcd_major(int b, int nblocks)
     return (b % 8) * per + b / 8;
 }
 
+// A batch of independent runs of one padded size: run r's arrays start at
+// base + r * stride; nu and the device scalars are per run.
+struct Batch {
+    int nruns;
+    long long sW; // compact omega (Wn, Ws), elements per run
+    long long sG; // each G field, elements per run
+    long long sH; // each H field, elements per run
+    const double *nu; // per-run viscosity, or null: StageArgs::nu for all
+};
+
+// The kernels are written as bodies over (block index blk of nblk blocks,
+// the block's LDS); the __global__ wrappers below give them one size, the
+// ensemble kernels (kernels_ensemble.hpp) select a size per block.
+
 // ------------------------------------------------------------- pass A
 template <typename T, int NP>
-__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
-k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
-         cpx<T> *__restrict__ Gw, FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc,
-         int reset_max)
+__device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
+                                           const cpx<T> *__restrict__ Wsrc,
+                                           cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
+                                           cpx<T> *__restrict__ Gw, const FusedDims &d,
+                                           const cpx<T> *__restrict__ tw, DevScalars *sc,
+                                           int reset_max, const Batch &bt)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     constexpr int E = C::E, TG = C::TG;
-    __shared__ cpx<T> lds[C::GPB * C::SLICE];
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
     // field-major jobs, XCD-major blocks: the blocks in flight on one XCD
     // hold adjacent kx of one field and fill whole lines of the G rows
-    const int job = xcd_major(blockIdx.x, gridDim.x) * C::GPB + g;
-    const bool active = job < 3 * d.K;
-    const int f = active ? job / d.K : 0, kx = active ? job % d.K : 0;
-    if (reset_max && blockIdx.x == 0 && threadIdx.x == 0) {
-        sc->vmax_bits = 0ull;
-        sc->bad = 0u;
+    const int job = xcd_major(blk, nblk) * C::GPB + g;
+    const bool active = job < 3 * d.K * bt.nruns;
+    const int run = active ? job / (3 * d.K) : 0, rem = job - run * 3 * d.K;
+    const int f = active ? rem / d.K : 0, kx = active ? rem % d.K : 0;
+    if (reset_max && active && f == 0 && kx == 0 && t == 0) {
+        sc[run].vmax_bits = 0ull;
+        sc[run].bad = 0u;
     }
-    const cpx<T> *src = Wsrc + (size_t)kx * d.KP;
+    const cpx<T> *src = Wsrc + run * bt.sW + (size_t)kx * d.KP;
     const T qx = (T)kx;
     cpx<T> v[E];
 #pragma unroll
@@ -98,10 +115,22 @@ k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__res
     }
     group_fft<T, E, TG, 1>(v, lds + g * C::SLICE, t, tw);
     if (active) {
-        cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + kx;
+        cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx;
 #pragma unroll
         for (int s = 0; s < E; ++s) dst[(size_t)(t + TG * s) * d.KG] = v[s];
     }
+}
+
+template <typename T, int NP>
+__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
+k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
+         cpx<T> *__restrict__ Gw, FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc,
+         int reset_max, Batch bt)
+{
+    using C = WaveCfg<T, NP>;
+    __shared__ cpx<T> lds[C::GPB * C::SLICE];
+    passA_body<T, NP>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc, reset_max, bt);
 }
 
 // ------------------------------------------------------------- pass B
@@ -127,22 +156,30 @@ __device__ __forceinline__ cpx<T> hermitian_at(const cpx<T> *__restrict__ G, int
 }
 
 template <typename T, int NP>
-__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void
-k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
-         const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
-         FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc, int forward, int track_max)
+__device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds_all,
+                                           const cpx<T> *__restrict__ Gu,
+                                           const cpx<T> *__restrict__ Gv,
+                                           const cpx<T> *__restrict__ Gw,
+                                           cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
+                                           const FusedDims &d, const cpx<T> *__restrict__ tw,
+                                           DevScalars *sc, int forward, int track_max,
+                                           const Batch &bt)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     constexpr int E = C::E, TG = C::TG;
-    __shared__ cpx<T> lds_all[C::GPB * C::SLICE];
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
     cpx<T> *lds = lds_all + g * C::SLICE;
-    const int y = xcd_major(blockIdx.x, gridDim.x) * C::GPB + g;
-    const bool active = y < NP;
+    const int job = xcd_major(blk, nblk) * C::GPB + g;
+    const bool active = job < NP * bt.nruns;
+    const int run = active ? job / NP : 0, y = job - run * NP;
     const int yy = active ? y : 0;
     const int h = d.N / 2;
+    Gu += run * bt.sG;
+    Gv += run * bt.sG;
+    Gw += run * bt.sG;
+    Huw += run * bt.sH;
+    Hvw += run * bt.sH;
 
     // w of this row (the imaginary part of the result is the rounding of a
     // real transform and is dropped)
@@ -196,29 +233,46 @@ k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
         double o = __shfl_xor(dm, off);
         dm = (o > dm || o != o) ? o : dm;
     }
+    // every group of a block belongs to the same run (GPB divides NP)
     if ((threadIdx.x & 63) == 0)
-        atomicMax(&sc->vmax_bits, (unsigned long long)__double_as_longlong(dm));
+        atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
+}
+
+template <typename T, int NP>
+__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void
+k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
+         const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
+         FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc, int forward, int track_max,
+         Batch bt)
+{
+    using C = WaveCfg<T, NP>;
+    __shared__ cpx<T> lds_all[C::GPB * C::SLICE];
+    passB_body<T, NP>(blockIdx.x, gridDim.x, lds_all, Gu, Gv, Gw, Huw, Hvw, d, tw, sc, forward,
+                      track_max, bt);
 }
 
 // ------------------------------------------------------------- pass C
 template <typename T, int NP>
-__global__ __launch_bounds__((2 * WaveCfg<T, NP>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
-k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
-         const cpx<T> *Wst, cpx<T> *Wout, FusedDims d, const cpx<T> *__restrict__ tw,
-         const int *__restrict__ trunc_i1, DevScalars *sc, StageArgs a)
+__device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
+                                           const cpx<T> *__restrict__ Huw,
+                                           const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
+                                           const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
+                                           const cpx<T> *__restrict__ tw,
+                                           const int *__restrict__ trunc_i1, DevScalars *sc,
+                                           const StageArgs &a, const Batch &bt)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     constexpr int E = C::E, TG = C::TG;
-    __shared__ cpx<T> lds_all[2 * C::GPB * C::SLICE];
     const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK; // half 0: uw, 1: vw
     const int g = lane / TG, t = lane % TG;
     cpx<T> *mine = lds_all + (half * C::GPB + g) * C::SLICE;
     cpx<T> *other = lds_all + ((1 - half) * C::GPB + g) * C::SLICE;
-    const int kx = blockIdx.x * C::GPB + g;
-    const bool active = kx < d.K;
-    const cpx<T> *src = (half == 0 ? Huw : Hvw) + (size_t)(active ? kx : 0) * NP;
+    const int job = blk * C::GPB + g;
+    const bool active = job < d.K * bt.nruns;
+    const int run = active ? job / d.K : 0, kx = active ? job - run * d.K : 0;
+    const cpx<T> *src = (half == 0 ? Huw : Hvw) + run * bt.sH + (size_t)kx * NP;
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
@@ -229,12 +283,13 @@ k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const c
         if ((s & 1) != half) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
     __syncthreads();
 
-    const double dtd = stage_dt(a, sc);
+    const double dtd = stage_dt(a, sc + run);
     const T dt = (T)dtd;
     const T qx = (T)kx;
+    const double nu = bt.nu ? bt.nu[run] : a.nu;
     const T inv_norm = (T)(1.0 / a.norm); // multiplications instead of the
                                           // reference's divisions: <= 1 ulp
-    const size_t col = (size_t)(active ? kx : 0) * d.KP;
+    const size_t col = run * bt.sW + (size_t)kx * d.KP;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         if ((s & 1) != half || !active) continue;
@@ -251,7 +306,7 @@ k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const c
             const cpx<T> Q = mk<T>(Qr.re * inv_norm, Qr.im * inv_norm);
             const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
             const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
-            const T sn = (T)a.nu * (qx * qx + qy * qy);
+            const T sn = (T)nu * (qx * qx + qy * qy);
             const cpx<T> ws = Wst[col + c];
             const cpx<T> dW = mk<T>((-du.re - dv.re) - sn * ws.re, (-du.im - dv.im) - sn * ws.im);
             const cpx<T> w0 = Wn[col + c];
@@ -272,11 +327,23 @@ k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const c
         }
         Wout[col + c] = out;
     }
-    if (a.stage == 3 && blockIdx.x == 0 && threadIdx.x == 0) {
-        sc->dt = dtd;
-        sc->time = sc->time + dtd;
-        sc->nadv = sc->nadv + 1;
+    if (a.stage == 3 && active && kx == 0 && half == 0 && t == 0) {
+        sc[run].dt = dtd;
+        sc[run].time = sc[run].time + dtd;
+        sc[run].nadv = sc[run].nadv + 1;
     }
+}
+
+template <typename T, int NP>
+__global__ __launch_bounds__((2 * WaveCfg<T, NP>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
+k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
+         const cpx<T> *Wst, cpx<T> *Wout, FusedDims d, const cpx<T> *__restrict__ tw,
+         const int *__restrict__ trunc_i1, DevScalars *sc, StageArgs a, Batch bt)
+{
+    using C = WaveCfg<T, NP>;
+    __shared__ cpx<T> lds_all[2 * C::GPB * C::SLICE];
+    passC_body<T, NP>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw, trunc_i1, sc, a, bt);
 }
 
 } // namespace s2d

@@ -7,6 +7,7 @@
 // stream; every entry point synchronises that stream before returning.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "kernels_fused.hpp"
+#include "kernels_ensemble.hpp"
 #include "kernels_fused_v2.hpp"
 #include "kernels_ref.hpp"
 #include "s2d_internal.hpp"
@@ -463,23 +465,24 @@ FusedDims fdims(const s2d_ctx *c)
 }
 
 template <typename T>
-int op_pack(s2d_ctx *c)
+int op_pack(s2d_ctx *c, int run = 0)
 {
     FusedDims d = fdims(c);
     size_t n = (size_t)d.K * d.KP;
     hipLaunchKernelGGL(k_pack<T>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
-                       F<T>(c, S2D_FIELD_W), (cpx<T> *)c->Wn, d, c->d_trunc);
+                       F<T>(c, S2D_FIELD_W), (cpx<T> *)c->Wn + run * c->sW, d, c->d_trunc);
     S2D_LAUNCHED(c, "k_pack");
     return S2D_OK;
 }
 
 template <typename T>
-int op_unpack(s2d_ctx *c)
+int op_unpack(s2d_ctx *c, int run = 0)
 {
     FusedDims d = fdims(c);
     size_t n = nc_of(c);
     hipLaunchKernelGGL(k_unpack<T>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
-                       (const cpx<T> *)c->Wn, F<T>(c, S2D_FIELD_W), F<T>(c, S2D_FIELD_PSI), d);
+                       (const cpx<T> *)c->Wn + run * c->sW, F<T>(c, S2D_FIELD_W),
+                       F<T>(c, S2D_FIELD_PSI), d);
     S2D_LAUNCHED(c, "k_unpack");
     return S2D_OK;
 }
@@ -589,7 +592,20 @@ int fused_steps_E(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
     return S2D_OK;
 }
 
-// ---- wave-FFT kernels for the compiled padded sizes
+// ---- lane-group FFT kernels for the compiled padded sizes
+
+// The context's runs as a kernel batch (per-run viscosity for an ensemble
+// size class; a plain context takes nu from the call).
+Batch batch_of(const s2d_ctx *c)
+{
+    Batch b;
+    b.nruns = c->nruns;
+    b.sW = c->sW;
+    b.sG = c->sG;
+    b.sH = c->sH;
+    b.nu = c->member ? c->d_nu : nullptr;
+    return b;
+}
 
 template <typename T, int NP>
 int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
@@ -597,11 +613,11 @@ int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
     ProfScope ps(c, S2D_KERNEL_PASS_A);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    int blocks = (3 * d.K + Cf::GPB - 1) / Cf::GPB;
+    int blocks = (3 * d.K * c->nruns + Cf::GPB - 1) / Cf::GPB;
     blocks = (blocks + 7) / 8 * 8; // a multiple of 8 for the XCD-major order
     hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(Cf::BLK), 0, c->stream, src,
                        (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2], d,
-                       (const cpx<T> *)c->col.tw, c->d_sc, reset);
+                       (const cpx<T> *)c->col.tw, c->d_sc, reset, batch_of(c));
     S2D_LAUNCHED(c, "k2_passA");
     return S2D_OK;
 }
@@ -612,11 +628,11 @@ int launch2_passB(s2d_ctx *c, int forward, int track_max)
     ProfScope ps(c, S2D_KERNEL_PASS_B);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    const int blocks = (NP + Cf::GPB - 1) / Cf::GPB;
+    const int blocks = (NP * c->nruns + Cf::GPB - 1) / Cf::GPB;
     hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(Cf::BLK), 0, c->stream,
                        (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2],
                        (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d, (const cpx<T> *)c->rowf.tw,
-                       c->d_sc, forward, track_max);
+                       c->d_sc, forward, track_max, batch_of(c));
     S2D_LAUNCHED(c, "k2_passB");
     return S2D_OK;
 }
@@ -627,10 +643,11 @@ int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &
     ProfScope ps(c, S2D_KERNEL_PASS_C);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    const int blocks = (d.K + Cf::GPB - 1) / Cf::GPB;
+    const int blocks = (d.K * c->nruns + Cf::GPB - 1) / Cf::GPB;
     hipLaunchKernelGGL((k2_passC<T, NP>), dim3(blocks), dim3(2 * Cf::BLK), 0, c->stream,
                        (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
-                       Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a);
+                       Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a,
+                       batch_of(c));
     S2D_LAUNCHED(c, "k2_passC");
     return S2D_OK;
 }
@@ -763,6 +780,47 @@ void wrote(s2d_ctx *c, int f)
     if (f == S2D_FIELD_W) c->compact_stale = true;
 }
 
+// Diagnostics of the W slot (reference layout) with run r's time and step
+// count; the caller has brought the W slot up to date.
+int diag_of(s2d_ctx *c, int run, s2d_diag *out)
+{
+    const size_t n = nc_of(c);
+    int blocks = grid_blocks(n, 256);
+    if (blocks > c->n_red) blocks = c->n_red;
+    if (c->precision == S2D_FP64)
+        hipLaunchKernelGGL(k_diag_partial<double>, dim3(blocks), dim3(256), 0, c->stream,
+                           F<double>(c, S2D_FIELD_W), c->g.cISize, c->g.nyG, c->d_kx, c->d_ky,
+                           c->g.kxMG, c->d_red);
+    else
+        hipLaunchKernelGGL(k_diag_partial<float>, dim3(blocks), dim3(256), 0, c->stream,
+                           F<float>(c, S2D_FIELD_W), c->g.cISize, c->g.nyG, c->d_kx, c->d_ky,
+                           c->g.kxMG, c->d_red);
+    S2D_LAUNCHED(c, "k_diag_partial");
+    std::vector<double> part((size_t)blocks * 5);
+    S2D_HIP(c, hipMemcpyAsync(part.data(), c->d_red, part.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    DevScalars sc;
+    S2D_HIP(c, hipMemcpyAsync(&sc, c->d_sc + run, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    S2D_TRY(sync(c));
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < blocks; ++b)
+        for (int q = 0; q < 5; ++q) {
+            double x = part[(size_t)b * 5 + q];
+            if (q == 3)
+                a[q] = x > a[q] ? x : a[q];
+            else
+                a[q] += x;
+        }
+    out->tke = a[0];
+    out->enstrophy = a[1];
+    out->palinstrophy = a[2];
+    out->max_abs_w = a[3];
+    out->time = sc.time;
+    out->nadv = sc.nadv;
+    if (a[4] > 0.0) return fail(c, S2D_ERR_NONFINITE, "non-finite vorticity in the state");
+    return S2D_OK;
+}
+
 int check_ctx(const s2d_ctx *c)
 {
     if (!c) return fail(nullptr, S2D_ERR_INVALID_ARG, "null context");
@@ -783,6 +841,59 @@ int set_device(s2d_ctx *c)
 }
 
 } // namespace
+
+namespace {
+int create_ctx(int nx, int ny, int dealias, int precision, int nruns, hipStream_t stream,
+               s2d_ctx **out);
+
+// The seeded decaying-turbulence recipe (kept modes 0 < |k| <= kMax,
+// E(k) ~ k^4 exp(-(k/k0)^2), |u_k|^2 = E/(pi k), phase 2 pi U(seed, kx, ky),
+// Hermitian on kx = 0, TKE with the ComputeSpectrum weights scaled to ke).
+// emit(i, j, re, im) receives every kept mode of the reference layout
+// (i = kx, j = FFTW row), scaled, in row-major order, the order in which the
+// TKE is summed: every caller gets the same bits.
+template <typename Emit>
+void turbulence_ic(const s2d_ctx *c, unsigned long long seed, double k0, double ke, Emit emit,
+                   int ny)
+{
+    const double PI = std::acos(-1.0);
+    struct Mode {
+        int i, j;
+        double re, im;
+    };
+    std::vector<Mode> modes;
+    double tke = 0.0;
+    const int kmax_i = c->g.cISize;
+    for (int j = 0; j < ny; ++j) {
+        const int ky = c->ky[j];
+        for (int i = 0; i < kmax_i; ++i) {
+            const int kx = c->kx[i];
+            if (kx == 0 && ky == 0) continue;
+            const double k2 = (double)kx * kx + (double)ky * ky;
+            const double km = std::sqrt(k2);
+            if (km > c->g.kMax + 1.0e-10) continue;
+            int sy = ky;
+            bool conj = false;
+            if (kx == 0 && ky < 0) {
+                sy = -ky;
+                conj = true;
+            }
+            const double q = km / k0;
+            const double E = k2 * k2 * std::exp(-q * q);
+            const double au = std::sqrt(E / (PI * km)); // |u_k|
+            const double th = 2.0 * PI * hash_uniform(seed, kx, sy);
+            const double amp = au * km; // |w_k|
+            double sn, cs; // glibc sincos: the same bits whichever compiler builds this
+            ::sincos(th, &sn, &cs);
+            modes.push_back({i, j, amp * cs, conj ? -(amp * sn) : amp * sn});
+            const double wt = (kx == 0 || kx == c->g.kxMG) ? 1.0 : 2.0;
+            tke += wt * 0.5 * au * au;
+        }
+    }
+    const double s = (tke > 0.0) ? std::sqrt(ke / tke) : 0.0;
+    for (const Mode &m : modes) emit(m.i, m.j, m.re * s, m.im * s);
+}
+}
 
 // ================================================================ C ABI
 
@@ -842,6 +953,16 @@ int s2d_wavenumbers(const s2d_grid *g, int *kxG, int *kyG)
 
 int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
 {
+    return create_ctx(nx, ny, dealias, precision, 1, nullptr, out);
+}
+
+} // extern "C"
+
+namespace {
+
+int create_ctx(int nx, int ny, int dealias, int precision, int nruns, hipStream_t stream,
+               s2d_ctx **out)
+{
     if (!out) return fail(nullptr, S2D_ERR_INVALID_ARG, "null output pointer");
     *out = nullptr;
     if (precision != S2D_FP64 && precision != S2D_FP32)
@@ -859,9 +980,11 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, S2D_ERR_NO_DEVICE, "no GPU visible");
 
+    if (nruns < 1) return fail(nullptr, S2D_ERR_INVALID_ARG, "run count must be positive");
     s2d_ctx *c = new s2d_ctx();
     c->g = g;
     c->precision = precision;
+    c->nruns = nruns;
     c->celt = precision == S2D_FP64 ? 16 : 8;
     auto bail = [&](int code) {
         g_err = c->err;
@@ -882,7 +1005,10 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
         const char *e = std::getenv("S2D_GENERIC_KERNELS");
         c->wave_kernels = !(e && e[0] == '1');
     }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (stream) {
+        c->stream = stream;
+        c->own_stream = false;
+    } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(S2D_ERR_HIP);
     }
@@ -924,7 +1050,8 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
     if (hipMalloc(&c->d_trunc, sizeof(int) * g.nyG) != hipSuccess ||
         hipMalloc(&c->d_kx, sizeof(int) * g.cISize) != hipSuccess ||
         hipMalloc(&c->d_ky, sizeof(int) * g.nyG) != hipSuccess ||
-        hipMalloc(&c->d_sc, sizeof(DevScalars)) != hipSuccess) {
+        hipMalloc(&c->d_sc, sizeof(DevScalars) * nruns) != hipSuccess ||
+        hipMalloc(&c->d_nu, sizeof(double) * nruns) != hipSuccess) {
         c->err = "device allocation of tables failed";
         return bail(S2D_ERR_ALLOC);
     }
@@ -934,9 +1061,13 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
         c->err = "upload of tables failed";
         return bail(S2D_ERR_HIP);
     }
-    DevScalars sc0;
-    std::memset(&sc0, 0, sizeof(sc0));
-    if (hipMemcpy(c->d_sc, &sc0, sizeof(sc0), hipMemcpyHostToDevice) != hipSuccess) {
+    std::vector<DevScalars> sc0((size_t)nruns);
+    std::memset(sc0.data(), 0, sizeof(DevScalars) * nruns);
+    std::vector<double> nu0((size_t)nruns, 0.0);
+    if (hipMemcpy(c->d_sc, sc0.data(), sizeof(DevScalars) * nruns, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->d_nu, nu0.data(), sizeof(double) * nruns, hipMemcpyHostToDevice) !=
+            hipSuccess) {
         c->err = "upload of scalars failed";
         return bail(S2D_ERR_HIP);
     }
@@ -959,25 +1090,33 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
         c->K = nx / 2 + 1;
         c->KP = ((nx + 1) + 7) / 8 * 8;
         c->KG = (c->K + 15) / 16 * 16;
-        const size_t ncomp = (size_t)c->K * c->KP * c->celt;
-        const size_t nmix = (size_t)c->KG * g.nxG * c->celt; // G: Np rows of KG; H: K columns of Np
+        c->sW = (long long)c->K * c->KP;  // compact column-major kept modes
+        c->sG = (long long)g.nxG * c->KG; // G: Np rows of KG
+        c->sH = (long long)c->K * g.nxG;  // H: K columns of Np
+        const size_t ncomp = (size_t)c->sW * nruns * c->celt;
+        const size_t nG = (size_t)c->sG * nruns * c->celt, nH = (size_t)c->sH * nruns * c->celt;
         if (hipMalloc(&c->Wn, ncomp) || hipMalloc(&c->Ws, ncomp)) {
             c->err = "device allocation of the compact state failed";
             return bail(S2D_ERR_ALLOC);
         }
         for (int f = 0; f < 3; ++f)
-            if (hipMalloc(&c->G[f], nmix)) {
+            if (hipMalloc(&c->G[f], nG)) {
                 c->err = "device allocation of the mixed fields failed";
                 return bail(S2D_ERR_ALLOC);
             }
         for (int f = 0; f < 2; ++f)
-            if (hipMalloc(&c->H[f], nmix)) {
+            if (hipMalloc(&c->H[f], nH)) {
                 c->err = "device allocation of the mixed fields failed";
                 return bail(S2D_ERR_ALLOC);
             }
     }
-    c->n_red = 512;
-    if (hipMalloc(&c->d_red, sizeof(double) * 5 * c->n_red)) {
+    if (nruns > 1 && !(c->fused && c->wave_kernels && c->g.nxG % 48 == 0 &&
+                       (c->g.nxG / 48 & (c->g.nxG / 48 - 1)) == 0 && c->g.nxG <= 3072)) {
+        c->err = "batched runs need a padded size with compiled kernels (N = 64..2048, powers of 2)";
+        return bail(S2D_ERR_UNSUPPORTED);
+    }
+    c->n_red = 512 > nruns ? 512 : nruns;
+    if (hipMalloc(&c->d_red, sizeof(double) * 6 * c->n_red)) {
         c->err = "device allocation of reduction buffers failed";
         return bail(S2D_ERR_ALLOC);
     }
@@ -990,6 +1129,10 @@ int s2d_create(int nx, int ny, int dealias, int precision, s2d_ctx **out)
     return S2D_OK;
 }
 
+} // namespace
+
+extern "C" {
+
 int s2d_destroy(s2d_ctx *c)
 {
     if (!c) return S2D_OK;
@@ -997,10 +1140,11 @@ int s2d_destroy(s2d_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     for (int f = 0; f < S2D_NFIELDS; ++f) (void)hipFree(c->field[f]);
-    void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->col.tw, c->rowh.tw,
-                  c->rowf.tw, c->Wn, c->Ws, c->G[0], c->G[1], c->G[2], c->H[0], c->H[1]};
+    void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->d_nu, c->col.tw,
+                  c->rowh.tw, c->rowf.tw, c->Wn, c->Ws, c->G[0], c->G[1], c->G[2], c->H[0],
+                  c->H[1]};
     for (void *p : ps) (void)hipFree(p);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return S2D_OK;
 }
@@ -1232,10 +1376,10 @@ int s2d_rk3_steps(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
         const size_t n = (size_t)c->K * c->KP;
         if (c->precision == S2D_FP64)
             hipLaunchKernelGGL(k_check_finite<double>, dim3(grid_blocks(n, 256)), dim3(256), 0,
-                               c->stream, (const cpx<double> *)c->Wn, n, c->d_sc);
+                               c->stream, (const cpx<double> *)c->Wn, n, (long long)n, c->d_sc);
         else
             hipLaunchKernelGGL(k_check_finite<float>, dim3(grid_blocks(n, 256)), dim3(256), 0,
-                               c->stream, (const cpx<float> *)c->Wn, n, c->d_sc);
+                               c->stream, (const cpx<float> *)c->Wn, n, (long long)n, c->d_sc);
         S2D_LAUNCHED(c, "k_check_finite");
     }
     DevScalars sc;
@@ -1292,41 +1436,7 @@ int s2d_diagnostics(s2d_ctx *c, s2d_diag *out)
     if (!out) return fail(c, S2D_ERR_INVALID_ARG, "null argument");
     S2D_TRY(set_device(c));
     S2D_TRY(ensure_ref(c));
-    const size_t n = nc_of(c);
-    int blocks = grid_blocks(n, 256);
-    if (blocks > c->n_red) blocks = c->n_red;
-    if (c->precision == S2D_FP64)
-        hipLaunchKernelGGL(k_diag_partial<double>, dim3(blocks), dim3(256), 0, c->stream,
-                           F<double>(c, S2D_FIELD_W), c->g.cISize, c->g.nyG, c->d_kx, c->d_ky,
-                           c->g.kxMG, c->d_red);
-    else
-        hipLaunchKernelGGL(k_diag_partial<float>, dim3(blocks), dim3(256), 0, c->stream,
-                           F<float>(c, S2D_FIELD_W), c->g.cISize, c->g.nyG, c->d_kx, c->d_ky,
-                           c->g.kxMG, c->d_red);
-    S2D_LAUNCHED(c, "k_diag_partial");
-    std::vector<double> part((size_t)blocks * 5);
-    S2D_HIP(c, hipMemcpyAsync(part.data(), c->d_red, part.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, c->stream));
-    DevScalars sc;
-    S2D_HIP(c, hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
-    S2D_TRY(sync(c));
-    double a[5] = {0, 0, 0, 0, 0};
-    for (int b = 0; b < blocks; ++b)
-        for (int q = 0; q < 5; ++q) {
-            double x = part[(size_t)b * 5 + q];
-            if (q == 3)
-                a[q] = x > a[q] ? x : a[q];
-            else
-                a[q] += x;
-        }
-    out->tke = a[0];
-    out->enstrophy = a[1];
-    out->palinstrophy = a[2];
-    out->max_abs_w = a[3];
-    out->time = sc.time;
-    out->nadv = sc.nadv;
-    if (a[4] > 0.0) return fail(c, S2D_ERR_NONFINITE, "non-finite vorticity in the state");
-    return S2D_OK;
+    return diag_of(c, 0, out);
 }
 
 int s2d_set_time(s2d_ctx *c, double time, long long nadv)
@@ -1347,41 +1457,13 @@ int s2d_set_turbulence(s2d_ctx *c, unsigned long long seed, double k0, double ke
     S2D_TRY(check_ctx(c));
     if (!(k0 > 0.0) || !(ke >= 0.0)) return fail(c, S2D_ERR_INVALID_ARG, "k0 and ke must be positive");
     S2D_TRY(set_device(c));
-    const double PI = std::acos(-1.0);
     const int cI = c->g.cISize, ny = c->g.nyG;
-    const size_t nd = 2 * nc_of(c);
-    std::vector<double> w(nd, 0.0);
-    double tke = 0.0;
-    for (int j = 0; j < ny; ++j) {
-        const int ky = c->ky[j];
-        for (int i = 0; i < cI; ++i) {
-            const int kx = c->kx[i];
-            if (kx == 0 && ky == 0) continue;
-            const double k2 = (double)kx * kx + (double)ky * ky;
-            const double km = std::sqrt(k2);
-            if (km > c->g.kMax + 1.0e-10) continue;
-            int sy = ky;
-            bool conj = false;
-            if (kx == 0 && ky < 0) {
-                sy = -ky;
-                conj = true;
-            }
-            const double q = km / k0;
-            const double E = k2 * k2 * std::exp(-q * q);
-            const double au = std::sqrt(E / (PI * km)); // |u_k|
-            const double th = 2.0 * PI * hash_uniform(seed, kx, sy);
-            const double amp = au * km; // |w_k|
-            const size_t k = 2 * ((size_t)i + (size_t)j * cI);
-            double sn, cs; // glibc sincos: the same bits whichever compiler builds this
-            ::sincos(th, &sn, &cs);
-            w[k] = amp * cs;
-            w[k + 1] = conj ? -(amp * sn) : amp * sn;
-            const double wt = (kx == 0 || kx == c->g.kxMG) ? 1.0 : 2.0;
-            tke += wt * 0.5 * au * au;
-        }
-    }
-    const double s = (tke > 0.0) ? std::sqrt(ke / tke) : 0.0;
-    for (size_t k = 0; k < nd; ++k) w[k] = w[k] * s;
+    std::vector<double> w(2 * nc_of(c), 0.0);
+    turbulence_ic(c, seed, k0, ke, [&](int i, int j, double re, double im) {
+        const size_t k = 2 * ((size_t)i + (size_t)j * cI);
+        w[k] = re;
+        w[k + 1] = im;
+    }, ny);
     S2D_TRY(s2d_set_field_host(c, S2D_FIELD_W, w.data()));
     S2D_TRY(S2D_DISPATCH_T(c, op_psi, S2D_FIELD_W, S2D_FIELD_PSI));
     return sync(c);
@@ -1451,3 +1533,5 @@ int s2d_profile_end(s2d_ctx *c, double *ms, long long *count)
 }
 
 } // extern "C"
+
+#include "s2d_ensemble.inc"

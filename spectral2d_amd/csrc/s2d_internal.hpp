@@ -60,6 +60,16 @@ struct s2d_ctx {
     s2d::DevFft rowf; // length nxG complex (rows, two-for-one in the fused pass)
     void *tw_x = nullptr; // exp(-2 pi i k / nxG), k = 0..nxG-1 (= rowf.tw)
 
+    // a context may carry a batch of independent runs of one size (the
+    // ensemble); run r's compact/G/H arrays start at base + r * stride and
+    // its device scalars are d_sc[r]. The reference-layout slots serve one
+    // run at a time.
+    int nruns = 1;
+    bool own_stream = true;
+    long long sW = 0, sG = 0, sH = 0; // elements per run
+    double *d_nu = nullptr;           // per-run viscosity (ensemble size classes)
+    bool member = false;              // a size class of an ensemble
+
     // fused compact state
     bool fused = false;
     bool wave_kernels = true; // compiled wave-FFT kernels for the hot sizes

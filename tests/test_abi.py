@@ -69,3 +69,15 @@ def test_null_context_is_an_error_not_a_crash():
     assert L.lib().s2d_rk3_step(None, 1e-3, 1e-3) == L.S2D_ERR_INVALID_ARG
     assert L.lib().s2d_transform_r2c(None, 0) == L.S2D_ERR_INVALID_ARG
     assert L.lib().s2d_destroy(None) == L.S2D_OK
+
+
+def test_ensemble_rejects_before_touching_a_device():
+    h = C.c_void_p()
+    for sizes, prec in (([128, 100], 1), ([128, 2048], 1), ([1024], 0), ([128], 7), ([], 1)):
+        arr = (C.c_int * max(1, len(sizes)))(*sizes)
+        rc = L.lib().s2d_ensemble_create(len(sizes), arr, prec, C.byref(h))
+        assert rc in (L.S2D_ERR_UNSUPPORTED, L.S2D_ERR_INVALID_ARG), (sizes, prec, rc)
+        assert not h.value
+    assert L.lib().s2d_ensemble_steps(None, 1, 0.0, 0.5, 0) == L.S2D_ERR_INVALID_ARG
+    assert L.lib().s2d_ensemble_destroy(None) == L.S2D_OK
+    assert b"null ensemble" in L.lib().s2d_ensemble_last_error(None)
