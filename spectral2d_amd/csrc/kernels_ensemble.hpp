@@ -7,8 +7,9 @@
 // blockIdx % 8, is also its XCD within the class), and each block runs the
 // pass body of its class's compiled size. The sizes served are those whose
 // lane group fits one wave (Np / E <= 64): every class then uses the same
-// 64-lane block (128 for pass C) and the same 64 * E elements of LDS, and
-// every block carries 64 * E points of work whatever its class.
+// blocks, EnsCfg::BLK_A / BLK_B lanes for passes A and B (BLK / TG lines)
+// and 128 for pass C, and every block carries the same number of points
+// whatever its class.
 #pragma once
 
 #include "kernels_fused_v2.hpp"
@@ -41,7 +42,15 @@ struct EnsArgs {
 template <typename T>
 struct EnsCfg {
     static constexpr int E = WaveCfg<T, 192>::E;
-    static constexpr int LDS = 64 * E; // GPB * SLICE of every served size
+    // lanes of a pass A / pass B block
+    static constexpr int BLK_A = 256, BLK_B = 256;
+    // lines per tile of passes A and B for padded size np
+    static constexpr int lpb_a(int np) { return BLK_A * E / np; }
+    static constexpr int lpb_b(int np) { return BLK_B * E / np; }
+    // the largest tiles of the served sizes (Np = 96)
+    static constexpr int LDS_A = TileCfg<T, 96, BLK_A * E / 96>::LDS;
+    static constexpr int LDS_B = TileCfg<T, 96, BLK_B * E / 96>::LDS;
+    static constexpr int LDS_C = 2 * 64 * E; // 2 * GPB * SLICE of every served size
 };
 
 // The padded sizes an ensemble class may have.
@@ -72,30 +81,32 @@ __device__ __forceinline__ int ens_class(const EnsArgs<T> &ea, int p, int &b)
     }
 
 template <typename T>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(EnsCfg<T>::BLK_A)
 __attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192>::WPE))) void
 k3_ens_passA(EnsArgs<T> ea, int stage)
 {
-    __shared__ cpx<T> lds[EnsCfg<T>::LDS];
+    __shared__ cpx<T> lds[EnsCfg<T>::LDS_A];
     int b = blockIdx.x;
     const EnsClass<T> &c = ea.c[ens_class(ea, 0, b)];
     const cpx<T> *src = stage == 1 ? c.Wn : c.Ws;
     S2D_ENS_SIZES(T, c.np,
-                  (passA_body<T, NP>(b, c.nblk[0], lds, src, c.G0, c.G1, c.G2, c.d, c.tw_col,
-                                     c.sc, stage == 1, c.bt)))
+                  (passA_body<T, NP, EnsCfg<T>::lpb_a(NP)>(b, c.nblk[0], lds, src, c.G0, c.G1,
+                                                          c.G2, c.d, c.tw_col, c.sc, stage == 1,
+                                                          c.bt)))
 }
 
 template <typename T>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(EnsCfg<T>::BLK_B)
 __attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192>::WPE_B))) void
 k3_ens_passB(EnsArgs<T> ea, int stage)
 {
-    __shared__ cpx<T> lds[EnsCfg<T>::LDS];
+    __shared__ cpx<T> lds[EnsCfg<T>::LDS_B];
     int b = blockIdx.x;
     const EnsClass<T> &c = ea.c[ens_class(ea, 1, b)];
     S2D_ENS_SIZES(T, c.np,
-                  (passB_body<T, NP>(b, c.nblk[1], lds, c.G0, c.G1, c.G2, c.H0, c.H1, c.d,
-                                     c.tw_row, c.sc, 1, stage == 1, c.bt)))
+                  (passB_body<T, NP, EnsCfg<T>::lpb_b(NP)>(b, c.nblk[1], lds, c.G0, c.G1, c.G2,
+                                                          c.H0, c.H1, c.d, c.tw_row, c.sc, 1,
+                                                          stage == 1, c.bt)))
 }
 
 template <typename T>
@@ -103,7 +114,7 @@ __global__ __launch_bounds__(128)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : WaveCfg<T, 192>::WPE))) void
 k3_ens_passC(EnsArgs<T> ea, int stage)
 {
-    __shared__ cpx<T> lds[2 * EnsCfg<T>::LDS];
+    __shared__ cpx<T> lds[EnsCfg<T>::LDS_C];
     int b = blockIdx.x;
     const EnsClass<T> &c = ea.c[ens_class(ea, 2, b)];
     StageArgs a = c.sa;
@@ -111,8 +122,8 @@ k3_ens_passC(EnsArgs<T> ea, int stage)
     const cpx<T> *src = stage == 1 ? c.Wn : c.Ws;
     cpx<T> *dst = stage == 3 ? c.Wn : c.Ws;
     S2D_ENS_SIZES(T, c.np,
-                  (passC_body<T, NP>(b, lds, c.H0, c.H1, c.Wn, src, dst, c.d, c.tw_col, c.trunc,
-                                     c.sc, a, c.bt)))
+                  (passC_body<T, NP>(b, lds, c.H0, c.H1, c.Wn, src, dst, c.d, c.tw_col, c.sc, a,
+                                     c.bt)))
 }
 
 } // namespace s2d

@@ -9,17 +9,23 @@
 // spills pass B and halves the waves in flight).
 //
 //  A  one lane group per (field, kx): u_hat, v_hat or w_hat of a column,
-//     inverse transform along y, stored into the rows of G (G[y*KG + kx]).
+//     inverse transform along y. A block holds LPB adjacent columns of one
+//     field; it stages them in LDS and stores the G rows (G[y*KG + kx]) as
+//     LPB-element contiguous segments.
 //  B  one lane group per row y (contiguous loads of its G rows): w from its
-//     Hermitian spectrum, u + i v
-//     (two-for-one), the products (u w + i v w) in registers, one forward
-//     transform, the uw / vw spectra separated from P_k and conj P_{Np-k}.
-//     Rows are assigned XCD-major so that the rows in flight on one XCD are
-//     adjacent and share the 128-byte lines of the (kx, y) fields in its L2.
+//     Hermitian spectrum, u + i v (two-for-one), the products (u w + i v w)
+//     in registers, one forward transform, the uw / vw spectra separated
+//     from P_k and conj P_{Np-k}. A block holds LPB adjacent rows and stores
+//     the H columns (H[kx*Np + y]) as LPB-element segments through LDS.
+//     Blocks are assigned XCD-major so that the tiles in flight on one XCD
+//     are adjacent and share the lines of the (kx, y) fields in its L2.
 //  C  two halves per column group: one transforms uw, the other vw; they
 //     swap halves of their results through LDS and finish the RHS,
-//     truncation and Shu-Osher stage update on the compact state.
+//     truncation and Shu-Osher stage update on the compact state (all its
+//     loads and stores are contiguous).
 #pragma once
+
+#include <type_traits>
 
 #include "fft_wave.hpp"
 #include "kernels_fused.hpp"
@@ -30,8 +36,12 @@ template <typename T, int NP>
 struct WaveCfg {
     static constexpr int E = 12;
     static constexpr int TG = NP / E;
-    static constexpr int GPB = TG >= 64 ? 1 : 64 / TG; // groups per block (A, B)
-    static constexpr int BLK = TG >= 64 ? TG : 64;      // threads per block (A, B)
+    // lines (columns / rows) per block of passes A and B, LPB * TG threads
+    static constexpr int LPB_A = TG >= 256 ? (sizeof(T) == 4 ? 2 : 1) : 256 / TG;
+    static constexpr int LPB_B = TG >= 128 && sizeof(T) == 8 ? 1 : TG >= 256 ? 2 : 256 / TG;
+    // pass C: column groups per half, two halves per block
+    static constexpr int GPB = TG >= 64 ? 1 : 64 / TG;
+    static constexpr int BLK = TG >= 64 ? TG : 64; // threads per half
     static constexpr int SLICE = LdsPad<T>::size(NP);
     static constexpr int WPE = sizeof(T) == 8 ? 3 : 4; // register budget, waves per SIMD
     // pass B holds w and u + i v; a multi-wave group needs more registers
@@ -39,6 +49,87 @@ struct WaveCfg {
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
                   "unsupported group FFT size");
 };
+
+// Tiles of passes A and B: LPB lines of NP elements, one lane group each.
+// The LDS slices are NP + 16 elements apart, so that the transposed reads
+// (LPB slices at one index) spread evenly over the banks.
+template <typename T, int NP, int LPB>
+struct TileCfg {
+    static constexpr int TG = WaveCfg<T, NP>::TG;
+    static constexpr int BLK = LPB * TG;
+    static constexpr int SL = NP + 16;
+    static constexpr int LDS = LPB * SL;
+    static_assert(LPB >= 1 && (LPB & (LPB - 1)) == 0 && BLK <= 1024, "bad tile");
+};
+
+// Component-wise select (a select of the whole struct can be lowered
+// through a private-memory array indexed by the condition).
+template <typename T>
+__device__ __forceinline__ cpx<T> sel(bool c, const cpx<T> &a, const cpx<T> &b)
+{
+    return mk<T>(c ? a.re : b.re, c ? a.im : b.im);
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, N).
+template <int B, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < N) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, N>(f);
+    }
+}
+
+// Where the padded index k = t + TG*S (0 <= t < TG) of a length-NP
+// transform falls under the 3/2 rule (h = N/2 = NP/3), for the whole lane
+// range of slot S: low (k <= h, kept ky >= 0 / kept kx), high
+// (k >= NP - h, kept ky < 0 / the conjugate side), zero (the padding), or
+// mixed (decided per lane). The loads of each slot are then unconditional,
+// so a lane's loads are all in flight together.
+enum PadRegion { PAD_LOW = 0, PAD_HIGH = 1, PAD_ZERO = 2, PAD_MIXED = 3 };
+
+template <int NP, int TG, int S>
+constexpr int pad_region()
+{
+    constexpr int h = NP / 3, lo = TG * S, hi = TG * S + TG - 1;
+    if (hi <= h) return PAD_LOW;
+    if (lo >= NP - h) return PAD_HIGH;
+    if (lo > h && hi < NP - h) return PAD_ZERO;
+    return PAD_MIXED;
+}
+
+// Compact row of padded index j (kept |ky| <= h), clamped to row 0 for
+// padding rows; *kept tells which.
+template <int NP, int TG, int S>
+__device__ __forceinline__ int compact_row_of(int j, bool *kept)
+{
+    constexpr int h = NP / 3, N = 2 * NP / 3;
+    constexpr int R = pad_region<NP, TG, S>();
+    if constexpr (R == PAD_LOW) {
+        *kept = true;
+        return j;
+    } else if constexpr (R == PAD_HIGH) {
+        *kept = true;
+        return j - NP + N + 1;
+    } else if constexpr (R == PAD_ZERO) {
+        *kept = false;
+        return 0;
+    } else {
+        const bool lo = j <= h, hi = j >= NP - h;
+        *kept = lo || hi;
+        return lo ? j : hi ? j - NP + N + 1 : 0;
+    }
+}
+
+// Signed wavenumber of padded (FFTW-order) index j.
+template <int NP, int TG, int S>
+__device__ __forceinline__ int signed_k(int j)
+{
+    constexpr int R = pad_region<NP, TG, S>();
+    if constexpr (R == PAD_LOW) return j;
+    else if constexpr (R == PAD_HIGH) return j - NP;
+    else return (j <= NP / 2) ? j : j - NP;
+}
 
 // XCD-major block order: the hardware deals blocks round-robin to 8 XCDs,
 // so block b runs on XCD b % 8; give each XCD a contiguous range of work.
@@ -64,7 +155,9 @@ struct Batch {
 // ensemble kernels (kernels_ensemble.hpp) select a size per block.
 
 // ------------------------------------------------------------- pass A
-template <typename T, int NP>
+// Jobs: (run, field, tile of LPB adjacent kx), field-major; tiles past K
+// hold inactive columns (zero input, nothing stored).
+template <typename T, int NP, int LPB>
 __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Wsrc,
                                            cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
@@ -74,36 +167,45 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
-    constexpr int E = C::E, TG = C::TG;
+    using Tl = TileCfg<T, NP, LPB>;
+    constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL;
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
-    // field-major jobs, XCD-major blocks: the blocks in flight on one XCD
-    // hold adjacent kx of one field and fill whole lines of the G rows
-    const int job = xcd_major(blk, nblk) * C::GPB + g;
-    const bool active = job < 3 * d.K * bt.nruns;
-    const int run = active ? job / (3 * d.K) : 0, rem = job - run * 3 * d.K;
-    const int f = active ? rem / d.K : 0, kx = active ? rem % d.K : 0;
-    if (reset_max && active && f == 0 && kx == 0 && t == 0) {
+    const int tiles = (d.K + LPB - 1) / LPB;
+    // XCD-major blocks: the tiles in flight on one XCD are adjacent
+    const int job = xcd_major(blk, nblk);
+    if (job >= 3 * tiles * bt.nruns) return; // the whole block
+    const int run = job / (3 * tiles), rem = job - run * 3 * tiles;
+    const int f = rem / tiles, kx0 = (rem - f * tiles) * LPB;
+    const int kx = kx0 + g;
+    const bool active = kx < d.K;
+    if (reset_max && f == 0 && kx0 == 0 && threadIdx.x == 0) {
         sc[run].vmax_bits = 0ull;
         sc[run].bad = 0u;
     }
-    const cpx<T> *src = Wsrc + run * bt.sW + (size_t)kx * d.KP;
+    const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
     const T qx = (T)kx;
     cpx<T> v[E];
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
+    static_for<0, E>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
         const int j = t + TG * s;
-        const int c = compact_row(j, d);
-        cpx<T> w = (active && c >= 0) ? src[c] : mk<T>((T)0, (T)0);
-        const int kyi = (j <= NP / 2) ? j : j - NP;
+        bool kept;
+        const int c = compact_row_of<NP, TG, s>(j, &kept);
+        kept = kept && active;
+        cpx<T> w = mk<T>((T)0, (T)0);
+        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
+            const cpx<T> x = src[c]; // in bounds for every lane
+            w = sel(kept, x, w);
+        }
+        const int kyi = signed_k<NP, TG, s>(j);
         const T qy = (T)kyi;
         if (f == 2) {
             v[s] = w;
         } else {
             cpx<T> ps = mk<T>((T)0, (T)0);
-            if (!(kx == 0 && kyi == 0) && c >= 0) {
+            if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
                 const T dd = (kx == 0) ? qy * qy : (kyi == 0) ? qx * qx : qx * qx + qy * qy;
-                const T inv = (T)1 / dd; // one reciprocal per mode
-                ps = mk<T>(w.re * inv, w.im * inv);
+                const T inv = (T)1 / (dd == (T)0 ? (T)1 : dd); // one reciprocal per mode
+                ps = sel(kx == 0 && kyi == 0, ps, mk<T>(w.re * inv, w.im * inv));
             }
             if (f == 0) {
                 v[s] = mk<T>(-(qy * ps.im), qy * ps.re); // II*ky*psi
@@ -112,51 +214,75 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
                 v[s] = mk<T>((T)-1 * r.re, (T)-1 * r.im);
             }
         }
-    }
-    group_fft<T, E, TG, 1>(v, lds + g * C::SLICE, t, tw);
-    if (active) {
-        cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx;
+    });
+    cpx<T> *slice = lds + g * SL;
+    group_fft<T, E, TG, 1>(v, slice, t, tw);
+    // transpose through LDS: the block's columns become LPB-element row segments
 #pragma unroll
-        for (int s = 0; s < E; ++s) dst[(size_t)(t + TG * s) * d.KG] = v[s];
+    for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx(t + TG * s)] = v[s];
+    __syncthreads();
+    cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) { // LPB * NP = E * BLK elements
+        const int e = threadIdx.x + BLK * m;
+        const int y = e / LPB, gi = e % LPB;
+        if (kx0 + gi < d.K) dst[(size_t)y * d.KG + gi] = lds[gi * SL + LdsPad<T>::idx(y)];
     }
 }
 
+// Blocks of pass A for nruns runs.
+template <typename T, int NP, int LPB>
+__host__ __device__ constexpr int passA_blocks(int K, int nruns)
+{
+    return 3 * ((K + LPB - 1) / LPB) * nruns;
+}
+
 template <typename T, int NP>
-__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
+__global__ __launch_bounds__((TileCfg<T, NP, WaveCfg<T, NP>::LPB_A>::BLK))
 __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
 k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
          cpx<T> *__restrict__ Gw, FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc,
          int reset_max, Batch bt)
 {
-    using C = WaveCfg<T, NP>;
-    __shared__ cpx<T> lds[C::GPB * C::SLICE];
-    passA_body<T, NP>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc, reset_max, bt);
+    constexpr int LPB = WaveCfg<T, NP>::LPB_A;
+    __shared__ cpx<T> lds[TileCfg<T, NP, LPB>::LDS];
+    passA_body<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc, reset_max,
+                           bt);
 }
 
 // ------------------------------------------------------------- pass B
 
-// Hermitian extension of one row's x spectrum at frequency k of a length-NP
-// transform (only Re of k = 0 is read).
-template <typename T>
-__device__ __forceinline__ cpx<T> hermitian_at(const cpx<T> *__restrict__ G, int k, int y,
-                                               const FusedDims &d)
+// Hermitian extension of one G row at frequency k = t + TG*S of a
+// length-NP transform: G[k] for k <= h, conj G[NP - k] for k >= NP - h,
+// zero between (only Re of k = 0 is read).
+template <typename T, int NP, int TG, int S>
+__device__ __forceinline__ cpx<T> herm_load(const cpx<T> *__restrict__ row, int t)
 {
-    const int h = d.N / 2;
-    if (k <= h) {
-        cpx<T> a = G[(size_t)y * d.KG + k];
-        if (k == 0) a.im = (T)0;
+    constexpr int h = NP / 3;
+    constexpr int R = pad_region<NP, TG, S>();
+    const int k = t + TG * S;
+    if constexpr (R == PAD_ZERO) {
+        return mk<T>((T)0, (T)0);
+    } else if constexpr (R == PAD_LOW) {
+        cpx<T> a = row[k];
+        if constexpr (S == 0) a.im = (t == 0) ? (T)0 : a.im;
         return a;
-    }
-    if (k >= d.Np - h) {
-        cpx<T> a = G[(size_t)y * d.KG + (d.Np - k)];
+    } else if constexpr (R == PAD_HIGH) {
+        cpx<T> a = row[NP - k];
         a.im = -a.im;
         return a;
+    } else {
+        const bool lo = k <= h, hi = k >= NP - h;
+        const cpx<T> x = row[lo ? k : hi ? NP - k : 0]; // in bounds for every lane
+        const T re = (lo || hi) ? x.re : (T)0;
+        const T im = lo ? x.im : hi ? -x.im : (T)0;
+        return mk<T>(re, k == 0 ? (T)0 : im);
     }
-    return mk<T>((T)0, (T)0);
 }
 
-template <typename T, int NP>
-__device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds_all,
+// Jobs: (run, tile of LPB adjacent rows).
+template <typename T, int NP, int LPB>
+__device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Gu,
                                            const cpx<T> *__restrict__ Gv,
                                            const cpx<T> *__restrict__ Gw,
@@ -167,38 +293,40 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds_all,
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
-    constexpr int E = C::E, TG = C::TG;
+    using Tl = TileCfg<T, NP, LPB>;
+    constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL;
+    constexpr int tiles = NP / LPB;
+    static_assert(NP % LPB == 0, "row tiles must divide the rows");
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
-    cpx<T> *lds = lds_all + g * C::SLICE;
-    const int job = xcd_major(blk, nblk) * C::GPB + g;
-    const bool active = job < NP * bt.nruns;
-    const int run = active ? job / NP : 0, y = job - run * NP;
-    const int yy = active ? y : 0;
-    const int h = d.N / 2;
+    const int job = xcd_major(blk, nblk);
+    if (job >= tiles * bt.nruns) return; // the whole block
+    const int run = job / tiles, y0 = (job - run * tiles) * LPB, y = y0 + g;
+    cpx<T> *slice = lds + g * SL;
     Gu += run * bt.sG;
     Gv += run * bt.sG;
     Gw += run * bt.sG;
-    Huw += run * bt.sH;
-    Hvw += run * bt.sH;
 
     // w of this row (the imaginary part of the result is the rounding of a
     // real transform and is dropped)
+    const cpx<T> *rw = Gw + (size_t)y * d.KG, *ru = Gu + (size_t)y * d.KG,
+                 *rv = Gv + (size_t)y * d.KG;
     cpx<T> v[E];
-#pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = hermitian_at(Gw, t + TG * s, yy, d);
-    group_fft<T, E, TG, 1>(v, lds, t, tw);
+    static_for<0, E>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        v[s] = herm_load<T, NP, TG, s>(rw, t);
+    });
+    group_fft<T, E, TG, 1>(v, slice, t, tw);
     T wr[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) wr[s] = v[s].re;
 
     // u + i v
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int k = t + TG * s;
-        const cpx<T> a = hermitian_at(Gu, k, yy, d), b = hermitian_at(Gv, k, yy, d);
+    static_for<0, E>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        const cpx<T> a = herm_load<T, NP, TG, s>(ru, t), b = herm_load<T, NP, TG, s>(rv, t);
         v[s] = mk<T>(a.re - b.im, a.im + b.re);
-    }
-    group_fft<T, E, TG, 1>(v, lds, t, tw);
+    });
+    group_fft<T, E, TG, 1>(v, slice, t, tw);
     T vmax = (T)0;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
@@ -207,59 +335,139 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds_all,
         v[s] = mk<T>(v[s].re * wr[s], v[s].im * wr[s]); // uw + i vw
     }
     if (forward) {
-        group_fft<T, E, TG, -1>(v, lds, t, tw);
+        group_fft<T, E, TG, -1>(v, slice, t, tw);
 #pragma unroll
-        for (int s = 0; s < E; ++s) lds[LdsPad<T>::idx(t + TG * s)] = v[s];
-        group_sync<TG>();
-        if (active) {
-#pragma unroll
-            for (int s = 0; s < E; ++s) {
-                const int k = t + TG * s;
-                if (k <= h) {
-                    const cpx<T> P = lds[LdsPad<T>::idx(k)];
-                    cpx<T> Pc = lds[LdsPad<T>::idx((NP - k) % NP)];
-                    Pc.im = -Pc.im;
-                    const cpx<T> su = add(P, Pc), df = sub(P, Pc);
-                    Huw[(size_t)k * NP + y] = mk<T>((T)0.5 * su.re, (T)0.5 * su.im);
-                    Hvw[(size_t)k * NP + y] = mk<T>((T)0.5 * df.im, (T)-0.5 * df.re);
-                }
+        for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx(t + TG * s)] = v[s];
+        __syncthreads();
+        // k <= N/2 = NP/3 of the block's LPB rows, rows fastest: LPB-element
+        // segments of the H columns
+        constexpr int H = NP / 3, CNT = (H + 1) * LPB;
+        Huw += run * bt.sH + y0;
+        Hvw += run * bt.sH + y0;
+#pragma unroll 1
+        for (int m = 0; m < (CNT + BLK - 1) / BLK; ++m) {
+            const int e = threadIdx.x + BLK * m;
+            if (e < CNT) {
+                const int k = e / LPB, r = e % LPB;
+                const cpx<T> *sl = lds + r * SL;
+                const cpx<T> P = sl[LdsPad<T>::idx(k)];
+                cpx<T> Pc = sl[LdsPad<T>::idx((NP - k) % NP)];
+                Pc.im = -Pc.im;
+                const cpx<T> su = add(P, Pc), df = sub(P, Pc);
+                Huw[(size_t)k * NP + r] = mk<T>((T)0.5 * su.re, (T)0.5 * su.im);
+                Hvw[(size_t)k * NP + r] = mk<T>((T)0.5 * df.im, (T)-0.5 * df.re);
             }
         }
     }
     if (!track_max) return;
-    double dm = active ? (double)vmax : 0.0;
+    double dm = (double)vmax;
     if (dm != dm) dm = __longlong_as_double(0x7ff8000000000000ll);
     for (int off = 32; off > 0; off >>= 1) {
         double o = __shfl_xor(dm, off);
         dm = (o > dm || o != o) ? o : dm;
     }
-    // every group of a block belongs to the same run (GPB divides NP)
+    // every row of a block belongs to the same run (LPB divides NP)
     if ((threadIdx.x & 63) == 0)
         atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
 }
 
+template <typename T, int NP, int LPB>
+__host__ __device__ constexpr int passB_blocks(int nruns)
+{
+    return NP / LPB * nruns;
+}
+
 template <typename T, int NP>
-__global__ __launch_bounds__((WaveCfg<T, NP>::BLK))
+__global__ __launch_bounds__((TileCfg<T, NP, WaveCfg<T, NP>::LPB_B>::BLK))
 __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void
 k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
          const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
          FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc, int forward, int track_max,
          Batch bt)
 {
-    using C = WaveCfg<T, NP>;
-    __shared__ cpx<T> lds_all[C::GPB * C::SLICE];
-    passB_body<T, NP>(blockIdx.x, gridDim.x, lds_all, Gu, Gv, Gw, Huw, Hvw, d, tw, sc, forward,
-                      track_max, bt);
+    constexpr int LPB = WaveCfg<T, NP>::LPB_B;
+    __shared__ cpx<T> lds[TileCfg<T, NP, LPB>::LDS];
+    passB_body<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc, forward,
+                           track_max, bt);
 }
 
 // ------------------------------------------------------------- pass C
+
+// The RHS, truncation and stage update of half HALF's rows (s with
+// s % 2 == HALF) of column kx; P / Q are the uw / vw column spectra (one of
+// them in registers, the other read from the partner half's LDS slice).
+template <typename T, int NP, int HALF>
+__device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E],
+                                             const cpx<T> *other, int t, int kx, bool active,
+                                             const cpx<T> *Wn, const cpx<T> *Wst, cpx<T> *Wout,
+                                             size_t col, const StageArgs &a, double dtd,
+                                             double nu)
+{
+#pragma clang fp contract(off)
+    using C = WaveCfg<T, NP>;
+    constexpr int E = C::E, TG = C::TG, H = NP / 3;
+    const T dt = (T)dtd;
+    const T qx = (T)kx;
+    const T inv_norm = (T)(1.0 / a.norm); // multiplications instead of the
+                                          // reference's divisions: <= 1 ulp
+    // loads first (unconditional, in bounds), so that they overlap
+    cpx<T> ws[E / 2], w0[E / 2];
+    int c[E / 2];
+    bool row[E / 2];
+    static_for<0, E / 2>([&](auto iI) {
+        constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
+        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
+            c[i] = compact_row_of<NP, TG, s>(t + TG * s, &row[i]);
+            ws[i] = Wst[col + c[i]];
+            w0[i] = Wn[col + c[i]];
+        }
+    });
+    static_for<0, E / 2>([&](auto iI) {
+        constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
+        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
+            const int j = t + TG * s;
+            const int kyi = signed_k<NP, TG, s>(j);
+            const T qy = (T)kyi;
+            // SetupTruncation's |k| <= kMax + 1e-10 (src/Spectral.F90:272-313)
+            // is kx^2 + ky^2 <= (N/2)^2 on integer wavenumbers
+            const bool keep = kx * kx + kyi * kyi <= H * H;
+            const cpx<T> o = other[LdsPad<T>::idx(j)];
+            const cpx<T> Pr = HALF == 0 ? v[s] : o, Qr = HALF == 0 ? o : v[s];
+            const cpx<T> P = mk<T>(Pr.re * inv_norm, Pr.im * inv_norm);
+            const cpx<T> Q = mk<T>(Qr.re * inv_norm, Qr.im * inv_norm);
+            const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
+            const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
+            const T sn = (T)nu * (qx * qx + qy * qy);
+            const cpx<T> wsv = ws[i], w0v = w0[i];
+            const cpx<T> dW =
+                mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
+            cpx<T> out;
+            if (a.stage == 1) {
+                out = mk<T>(w0v.re + dt * dW.re, w0v.im + dt * dW.im);
+            } else if (a.stage == 2) {
+                const T cc = (T)0.25 * dt;
+                out = mk<T>(((T)0.75 * w0v.re + (T)0.25 * wsv.re) + cc * dW.re,
+                            ((T)0.75 * w0v.im + (T)0.25 * wsv.im) + cc * dW.im);
+            } else {
+                const T cc = dt * (T)2;
+                // (w + 2 w2 + 2 dt dW)/3 with one correctly rounded division:
+                // a rounded 1/3 multiplier would bias |w| by its rounding
+                // error every step
+                out = mk<T>(((w0v.re + (T)2 * wsv.re) + cc * dW.re) / (T)3,
+                            ((w0v.im + (T)2 * wsv.im) + cc * dW.im) / (T)3);
+            }
+            out = sel(keep, out, mk<T>((T)0, (T)0));
+            if (active && row[i]) Wout[col + c[i]] = out;
+        }
+    });
+}
+
 template <typename T, int NP>
 __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
                                            const cpx<T> *__restrict__ Huw,
                                            const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
                                            const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
-                                           const cpx<T> *__restrict__ tw,
-                                           const int *__restrict__ trunc_i1, DevScalars *sc,
+                                           const cpx<T> *__restrict__ tw, DevScalars *sc,
                                            const StageArgs &a, const Batch &bt)
 {
 #pragma clang fp contract(off)
@@ -284,49 +492,12 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
     __syncthreads();
 
     const double dtd = stage_dt(a, sc + run);
-    const T dt = (T)dtd;
-    const T qx = (T)kx;
     const double nu = bt.nu ? bt.nu[run] : a.nu;
-    const T inv_norm = (T)(1.0 / a.norm); // multiplications instead of the
-                                          // reference's divisions: <= 1 ulp
     const size_t col = run * bt.sW + (size_t)kx * d.KP;
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-        if ((s & 1) != half || !active) continue;
-        const int j = t + TG * s;
-        const int c = compact_row(j, d);
-        if (c < 0) continue;
-        const int kyi = (j <= NP / 2) ? j : j - NP;
-        const T qy = (T)kyi;
-        cpx<T> out = mk<T>((T)0, (T)0);
-        if (kx < trunc_i1[j]) {
-            const cpx<T> o = other[LdsPad<T>::idx(j)];
-            const cpx<T> Pr = half == 0 ? v[s] : o, Qr = half == 0 ? o : v[s];
-            const cpx<T> P = mk<T>(Pr.re * inv_norm, Pr.im * inv_norm);
-            const cpx<T> Q = mk<T>(Qr.re * inv_norm, Qr.im * inv_norm);
-            const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
-            const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
-            const T sn = (T)nu * (qx * qx + qy * qy);
-            const cpx<T> ws = Wst[col + c];
-            const cpx<T> dW = mk<T>((-du.re - dv.re) - sn * ws.re, (-du.im - dv.im) - sn * ws.im);
-            const cpx<T> w0 = Wn[col + c];
-            if (a.stage == 1) {
-                out = mk<T>(w0.re + dt * dW.re, w0.im + dt * dW.im);
-            } else if (a.stage == 2) {
-                const T cc = (T)0.25 * dt;
-                out = mk<T>(((T)0.75 * w0.re + (T)0.25 * ws.re) + cc * dW.re,
-                            ((T)0.75 * w0.im + (T)0.25 * ws.im) + cc * dW.im);
-            } else {
-                const T cc = dt * (T)2;
-                // (w + 2 w2 + 2 dt dW)/3 with one correctly rounded division:
-                // a rounded 1/3 multiplier would bias |w| by its rounding
-                // error every step
-                out = mk<T>(((w0.re + (T)2 * ws.re) + cc * dW.re) / (T)3,
-                            ((w0.im + (T)2 * ws.im) + cc * dW.im) / (T)3);
-            }
-        }
-        Wout[col + c] = out;
-    }
+    if (half == 0) // wave-uniform
+        passC_finish<T, NP, 0>(v, other, t, kx, active, Wn, Wst, Wout, col, a, dtd, nu);
+    else
+        passC_finish<T, NP, 1>(v, other, t, kx, active, Wn, Wst, Wout, col, a, dtd, nu);
     if (a.stage == 3 && active && kx == 0 && half == 0 && t == 0) {
         sc[run].dt = dtd;
         sc[run].time = sc[run].time + dtd;
@@ -343,7 +514,7 @@ k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const c
 {
     using C = WaveCfg<T, NP>;
     __shared__ cpx<T> lds_all[2 * C::GPB * C::SLICE];
-    passC_body<T, NP>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw, trunc_i1, sc, a, bt);
+    passC_body<T, NP>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
 }
 
 } // namespace s2d

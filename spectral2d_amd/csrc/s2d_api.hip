@@ -611,13 +611,13 @@ template <typename T, int NP>
 int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
 {
     ProfScope ps(c, S2D_KERNEL_PASS_A);
-    using Cf = WaveCfg<T, NP>;
+    constexpr int LPB = WaveCfg<T, NP>::LPB_A;
     FusedDims d = fdims(c);
-    int blocks = (3 * d.K * c->nruns + Cf::GPB - 1) / Cf::GPB;
+    int blocks = passA_blocks<T, NP, LPB>(d.K, c->nruns);
     blocks = (blocks + 7) / 8 * 8; // a multiple of 8 for the XCD-major order
-    hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(Cf::BLK), 0, c->stream, src,
-                       (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2], d,
-                       (const cpx<T> *)c->col.tw, c->d_sc, reset, batch_of(c));
+    hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0,
+                       c->stream, src, (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2],
+                       d, (const cpx<T> *)c->col.tw, c->d_sc, reset, batch_of(c));
     S2D_LAUNCHED(c, "k2_passA");
     return S2D_OK;
 }
@@ -626,13 +626,14 @@ template <typename T, int NP>
 int launch2_passB(s2d_ctx *c, int forward, int track_max)
 {
     ProfScope ps(c, S2D_KERNEL_PASS_B);
-    using Cf = WaveCfg<T, NP>;
+    constexpr int LPB = WaveCfg<T, NP>::LPB_B;
     FusedDims d = fdims(c);
-    const int blocks = (NP * c->nruns + Cf::GPB - 1) / Cf::GPB;
-    hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(Cf::BLK), 0, c->stream,
-                       (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2],
-                       (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d, (const cpx<T> *)c->rowf.tw,
-                       c->d_sc, forward, track_max, batch_of(c));
+    int blocks = passB_blocks<T, NP, LPB>(c->nruns);
+    blocks = (blocks + 7) / 8 * 8;
+    hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0,
+                       c->stream, (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1],
+                       (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d,
+                       (const cpx<T> *)c->rowf.tw, c->d_sc, forward, track_max, batch_of(c));
     S2D_LAUNCHED(c, "k2_passB");
     return S2D_OK;
 }
