@@ -235,6 +235,13 @@ int s2d_ensemble_get_field(s2d_ensemble *ens, int run, int field, double *buf);
 int s2d_ensemble_steps(s2d_ensemble *ens, int nsteps, double dt, double cfl, int flags);
 /* out[nruns]: s2d_diagnostics of each run. */
 int s2d_ensemble_diagnostics(s2d_ensemble *ens, s2d_diag *out);
+/* Launch schedule of a step: S2D_ENS_UNIFIED (default), one launch per
+ * kernel pass covering every size class; S2D_ENS_PER_CLASS, one launch per
+ * pass and size class of the kernels compiled for that size alone. Both
+ * compute the same bits. */
+#define S2D_ENS_UNIFIED 0
+#define S2D_ENS_PER_CLASS 1
+int s2d_ensemble_set_schedule(s2d_ensemble *ens, int schedule);
 /* Kernel timing of the ensemble launches (eager steps only), as
  * s2d_profile_begin / s2d_profile_end. */
 int s2d_ensemble_profile_begin(s2d_ensemble *ens, int max_launches);

@@ -47,11 +47,14 @@ EXPORTS = (
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
-    "s2d_ensemble_steps", "s2d_ensemble_diagnostics", "s2d_ensemble_profile_begin",
+    "s2d_ensemble_steps", "s2d_ensemble_diagnostics", "s2d_ensemble_set_schedule",
+    "s2d_ensemble_profile_begin",
     "s2d_ensemble_profile_end",
 )
 
 ENS_GRAPH = 1
+ENS_UNIFIED = 0
+ENS_PER_CLASS = 1
 
 
 class Grid(C.Structure):
@@ -141,6 +144,7 @@ def lib():
     L.s2d_ensemble_get_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.s2d_ensemble_steps.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int]
     L.s2d_ensemble_diagnostics.argtypes = [vp, P(Diag)]
+    L.s2d_ensemble_set_schedule.argtypes = [vp, C.c_int]
     L.s2d_ensemble_profile_begin.argtypes = [vp, C.c_int]
     L.s2d_ensemble_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
     for name in EXPORTS:

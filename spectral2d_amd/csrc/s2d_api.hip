@@ -610,7 +610,6 @@ Batch batch_of(const s2d_ctx *c)
 template <typename T, int NP>
 int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
 {
-    ProfScope ps(c, S2D_KERNEL_PASS_A);
     constexpr int LPB = WaveCfg<T, NP>::LPB_A;
     FusedDims d = fdims(c);
     int blocks = passA_blocks<T, NP, LPB>(d.K, c->nruns);
@@ -625,7 +624,6 @@ int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
 template <typename T, int NP>
 int launch2_passB(s2d_ctx *c, int forward, int track_max)
 {
-    ProfScope ps(c, S2D_KERNEL_PASS_B);
     constexpr int LPB = WaveCfg<T, NP>::LPB_B;
     FusedDims d = fdims(c);
     int blocks = passB_blocks<T, NP, LPB>(c->nruns);
@@ -641,7 +639,6 @@ int launch2_passB(s2d_ctx *c, int forward, int track_max)
 template <typename T, int NP>
 int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &a)
 {
-    ProfScope ps(c, S2D_KERNEL_PASS_C);
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
     const int blocks = (d.K * c->nruns + Cf::GPB - 1) / Cf::GPB;
@@ -668,9 +665,18 @@ int fused_steps_wave(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
             const cpx<T> *src = stage == 1 ? Wn : Ws;
             cpx<T> *dst = stage == 3 ? Wn : Ws;
             a.stage = stage;
-            S2D_TRY((launch2_passA<T, NP>(c, src, stage == 1)));
-            S2D_TRY((launch2_passB<T, NP>(c, 1, stage == 1)));
-            S2D_TRY((launch2_passC<T, NP>(c, src, dst, a)));
+            {
+                ProfScope ps(c, S2D_KERNEL_PASS_A);
+                S2D_TRY((launch2_passA<T, NP>(c, src, stage == 1)));
+            }
+            {
+                ProfScope ps(c, S2D_KERNEL_PASS_B);
+                S2D_TRY((launch2_passB<T, NP>(c, 1, stage == 1)));
+            }
+            {
+                ProfScope ps(c, S2D_KERNEL_PASS_C);
+                S2D_TRY((launch2_passC<T, NP>(c, src, dst, a)));
+            }
         }
     }
     return S2D_OK;

@@ -101,6 +101,11 @@ class Ensemble:
         self._c(L.lib().s2d_ensemble_steps(self._h, int(nsteps), float(dt), float(cfl),
                                            L.ENS_GRAPH if graph else 0))
 
+    def set_schedule(self, per_class: bool):
+        """One launch per pass for all sizes (False) or per size class (True)."""
+        self._c(L.lib().s2d_ensemble_set_schedule(
+            self._h, L.ENS_PER_CLASS if per_class else L.ENS_UNIFIED))
+
     def diagnostics(self) -> list:
         d = (L.Diag * self.nruns)()
         self._c(L.lib().s2d_ensemble_diagnostics(self._h, d))
