@@ -12,6 +12,12 @@ Prints ONE JSON line. `value` = resolved Mpts x RK3 steps per second.
 library's stream around every launch in the timed region. `cpu_baseline`
 times the CPU oracle (a C restatement of the reference algorithm) on the
 host cores for a bounded number of steps of the same workload.
+
+The same line carries `ensemble`: the batched FP32 ensemble of 256
+independent decaying-turbulence runs (N drawn from {128, 256, 512}, k0 in
+[4, 16], nu log-uniform in [1e-4, 1e-3], seeds 1000 + m, per-run CFL 0.5),
+advanced by hipGraph replay of one captured step; its value is
+sum_r N_r^2 x steps / s, with the roofline of its dominant kernel.
 """
 from __future__ import annotations
 
@@ -44,7 +50,9 @@ def alg_bytes_per_launch(N: int, s: int):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
-KERNEL_TAGS = ("k_passA", "k_passB", "k_passC")
+# kernel-name patterns of the three passes per workload
+KERNEL_TAGS = {"c1": ("k2_passA<", "k2_passB<", "k2_passC<"),
+               "ensemble": ("k3_ens_passA<", "k3_ens_passB<", "k3_ens_passC<")}
 
 
 def kernel_source_hash() -> str:
@@ -59,50 +67,134 @@ def kernel_source_hash() -> str:
     return h.hexdigest()[:16]
 
 
-def pmc_summary(fetch_dir: str, write_dir: str, copy_mib: int, N: int, prec: str) -> dict:
-    """Per-dispatch HBM bytes of the fused kernels from two rocprofv3 --pmc
+def pmc_summary(fetch_dir: str, write_dir: str, copy_mib: int, key: str, tags) -> dict:
+    """Per-dispatch HBM bytes of the three passes from two rocprofv3 --pmc
     runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the
-    counter reads for the plain copy of copy_mib MiB in the same run."""
+    counter reads for the plain copy of copy_mib MiB in the same run; stored
+    under `key` in profiles/traffic.json."""
     import csv
     import glob
 
     def load(d, counter):
-        (path,) = glob.glob(os.path.join(d, "*counter_collection.csv"))
+        (path,) = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
         copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]
                   or "copy" in r["Kernel_Name"].lower()]
         factor = (copy_mib * 1024.0) / max(copies)
         per = {}
-        for tag in KERNEL_TAGS:
-            v = [float(r["Counter_Value"]) for r in rows if f"{tag}<" in r["Kernel_Name"]]
+        for tag in tags:
+            v = [float(r["Counter_Value"]) for r in rows if tag in r["Kernel_Name"]]
             per[tag] = sum(v) / len(v) * 1024.0 * factor if v else None
         return per, factor
 
     fetch, ff = load(fetch_dir, "FETCH_SIZE")
     write, wf = load(write_dir, "WRITE_SIZE")
-    out = {"source_hash": kernel_source_hash(), "N": N, "precision": prec,
-           "fetch_factor": ff, "write_factor": wf, "copy_mib": copy_mib,
-           "kernels": {t: {"read_bytes": fetch[t], "write_bytes": write[t],
-                           "traffic": (fetch[t] + write[t]) if fetch[t] and write[t] else None}
-                       for t in KERNEL_TAGS}}
-    os.makedirs(os.path.dirname(TRAFFIC_JSON), exist_ok=True)
-    with open(TRAFFIC_JSON, "w") as f:
-        json.dump(out, f, indent=1)
-    return out
-
-
-def measured_traffic(N: int, prec: str):
-    """Per-kernel traffic from profiles/traffic.json when it was measured on
-    these kernels and this workload; None otherwise."""
+    entry = {"source_hash": kernel_source_hash(), "fetch_factor": ff, "write_factor": wf,
+             "copy_mib": copy_mib,
+             "kernels": {t: {"read_bytes": fetch[t], "write_bytes": write[t],
+                             "traffic": (fetch[t] + write[t]) if fetch[t] and write[t] else None}
+                         for t in tags}}
     try:
         with open(TRAFFIC_JSON) as f:
-            t = json.load(f)
+            allt = json.load(f)
+        if "kernels" in allt:
+            allt = {}
     except (OSError, ValueError):
-        return None
-    if t.get("source_hash") != kernel_source_hash() or t.get("N") != N or \
-            t.get("precision") != prec:
-        return None
+        allt = {}
+    allt[key] = entry
+    os.makedirs(os.path.dirname(TRAFFIC_JSON), exist_ok=True)
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(allt, f, indent=1)
+    return entry
+
+
+def measured_traffic(key: str):
+    """Per-kernel traffic from profiles/traffic.json when it was measured on
+    these kernels and this workload; {} otherwise."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f).get(key)
+    except (OSError, ValueError, AttributeError):
+        return {}
+    if not t or t.get("source_hash") != kernel_source_hash():
+        return {}
     return {k: v.get("traffic") for k, v in t["kernels"].items()}
+
+
+def ensemble_draw(nruns: int = 256, seed: int = 2024):
+    """The C2 draw: sizes uniform from {128, 256, 512}, k0 ~ U[4, 16],
+    nu = 10^U[-4, -3], member m seeded 1000 + m."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 256, 512], nruns)
+    k0 = rng.uniform(4.0, 16.0, nruns)
+    nu = 10.0 ** rng.uniform(-4.0, -3.0, nruns)
+    seeds = np.arange(1000, 1000 + nruns, dtype=np.uint64)
+    return sizes, k0, nu, seeds
+
+
+def ens_alg_bytes(sizes, s: int):
+    """Per-launch algorithmic bytes of the ensemble passes (the sums over
+    runs of the per-run figures of alg_bytes_per_launch)."""
+    tot = [0, 0, 0]
+    for N in sizes:
+        for k, b in enumerate(alg_bytes_per_launch(int(N), s)):
+            tot[k] += b
+    return tuple(tot)
+
+
+def run_ensemble(steps: int, warmup: int, cfl: float):
+    import numpy as np
+    from spectral2d_amd.ensemble import Ensemble
+    from spectral2d_amd.spectral import FP32
+    sizes, k0, nu, seeds = ensemble_draw()
+    e = Ensemble(sizes, FP32)
+    e.set_viscosity(nu)
+    e.set_turbulence(seeds, k0, 0.5)
+    if warmup > 0:
+        e.steps(warmup, cfl=cfl, graph=True)
+    t0 = time.perf_counter()
+    e.steps(steps, cfl=cfl, graph=True)
+    el = time.perf_counter() - t0
+    # per-kernel durations: eager launches of the same step, events on the
+    # ensemble's stream (a graph replay records no per-kernel events)
+    kt = max(2, min(20, steps))
+    e.profile_begin(9 * kt)
+    e.steps(kt, cfl=cfl)
+    prof = e.profile_end()
+    d = e.diagnostics()
+    e.close()
+    per = ens_alg_bytes(sizes, 4)
+    avg_ms = [ms / max(n, 1) for ms, n in prof]
+    dom = max(range(3), key=lambda k: prof[k][0])
+    achieved = per[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    step_bytes = 3 * sum(per)
+    traffic = measured_traffic("ensemble")
+    tags = KERNEL_TAGS["ensemble"]
+    pts = float(np.sum(sizes.astype(np.float64) ** 2))
+    return {
+        "metric": "sum_r N_r^2 x RK3 steps / s (FP32 ensemble)",
+        "value": round(pts / 1e6 * steps / el, 3),
+        "unit": UNIT,
+        "steps": steps, "warmup": warmup,
+        "ms_per_step": round(el / steps * 1e3, 5),
+        "dtype": "f32",
+        "config": {"workload": "256 independent decaying-turbulence runs, N in {128, 256, 512} "
+                               "(counts %s), k0 U[4,16], nu log-U[1e-4,1e-3], per-run CFL 0.5, "
+                               "hipGraph replay" % [int((sizes == n).sum()) for n in (128, 256, 512)],
+                   "runs": int(len(sizes))},
+        "alg_bytes_per_step": step_bytes,
+        "alg_gbs": round(step_bytes / (el / steps) / 1e9, 1),
+        "hbm_frac_step": round(step_bytes / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic[tags[dom]]) if traffic.get(tags[dom]) else None},
+        "kernels": {KERNELS[k]: {"avg_us": round(avg_ms[k] * 1e3, 2), "alg_bytes": per[k],
+                                 "gbs": round(per[k] / (avg_ms[k] * 1e-3) / 1e9, 1),
+                                 "traffic": traffic.get(tags[k])} for k in range(3)},
+        "sim_time_range": [min(x["time"] for x in d), max(x["time"] for x in d)],
+    }
 
 
 def cpu_baseline(N: int, nu: float, cfl: float, budget_s: float):
@@ -151,11 +243,18 @@ def main(argv=None):
                     help="parse two rocprofv3 --pmc output directories (FETCH_SIZE, "
                          "WRITE_SIZE runs made with --copy-calibration) into "
                          "profiles/traffic.json and exit")
+    ap.add_argument("--workload", choices=("all", "c1", "ensemble"), default="all",
+                    help="c1: the 1024^2 run only; ensemble: the ensemble only (its JSON "
+                         "object is printed alone); all: both on one line")
+    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--ens-warmup", type=int, default=10)
     args = ap.parse_args(argv)
 
     if args.pmc_summary:
+        key = "ensemble" if args.workload == "ensemble" else f"c1_{args.precision}_{args.n}"
+        tags = KERNEL_TAGS["ensemble" if args.workload == "ensemble" else "c1"]
         out = pmc_summary(args.pmc_summary[0], args.pmc_summary[1],
-                          args.copy_calibration or 256, args.n, args.precision)
+                          args.copy_calibration or 256, key, tags)
         print(json.dumps(out, indent=1))
         return 0
 
@@ -175,6 +274,10 @@ def main(argv=None):
             b.copy_(a)
         torch.cuda.synchronize()
         print(f"copy calibration: 3 x {args.copy_calibration} MiB read + write", file=sys.stderr)
+
+    if args.workload == "ensemble":
+        print(json.dumps(run_ensemble(args.ens_steps, args.ens_warmup, args.cfl)), flush=True)
+        return 0
 
     N = args.n
     prec = FP64 if args.precision == "fp64" else FP32
@@ -201,15 +304,16 @@ def main(argv=None):
     avg_ms = [ms / max(n, 1) for ms, n in prof]
     dom = max(range(3), key=lambda k: prof[k][0])
     achieved = per[dom] / (avg_ms[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(N, args.precision) or {}
+    traffic = measured_traffic(f"c1_{args.precision}_{N}")
+    tags = KERNEL_TAGS["c1"]
     kernel_table = {
         KERNELS[k]: {"avg_us": round(avg_ms[k] * 1e3, 3), "launches": prof[k][1],
                      "alg_bytes": per[k],
                      "gbs": round(per[k] / (avg_ms[k] * 1e-3) / 1e9, 1) if avg_ms[k] else None,
-                     "traffic": traffic.get(KERNEL_TAGS[k])}
+                     "traffic": traffic.get(tags[k])}
         for k in range(3)
     }
-    dom_traffic = traffic.get(KERNEL_TAGS[dom])
+    dom_traffic = traffic.get(tags[dom])
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -233,6 +337,8 @@ def main(argv=None):
                      "traffic": round(dom_traffic) if dom_traffic else None},
         "kernels": kernel_table,
     }
+    if args.workload == "all":
+        out["ensemble"] = run_ensemble(args.ens_steps, args.ens_warmup, args.cfl)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(N, args.nu, args.cfl, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -5,6 +5,7 @@
 // and FP64-generated twiddle tables, device slots, and the per-routine and
 // fused-step entry points. All GPU work of a context goes to its own HIP
 // stream; every entry point synchronises that stream before returning.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -199,6 +200,37 @@ void allow_lds(K kern, size_t bytes)
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// While profiling is on, a ProfScope reserves an event pair for the next
+// fused launch of its context; launch() hands the pair to
+// hipExtLaunchKernelGGL, which records them at the kernel's own start and
+// end (no dispatch gaps inside the interval).
+struct ProfScope {
+    s2d_ctx *c;
+    int slot = -1;
+    ProfScope(s2d_ctx *c_, int kind) : c(c_)
+    {
+        if (c->prof_on && c->prof_used < (int)c->prof_kind.size()) {
+            slot = c->prof_used++;
+            c->prof_kind[slot] = kind;
+            c->prof_slot = slot;
+        }
+    }
+    ~ProfScope() { c->prof_slot = -1; }
+};
+
+template <typename K, typename... Args>
+void launch(s2d_ctx *c, K kern, dim3 grid, dim3 block, size_t shmem, Args... args)
+{
+    if (c->prof_slot >= 0) {
+        const int u = c->prof_slot;
+        c->prof_slot = -1;
+        hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)shmem, c->stream, c->prof_ev[2 * u],
+                              c->prof_ev[2 * u + 1], 0, args...);
+    } else {
+        hipLaunchKernelGGL(kern, grid, block, shmem, c->stream, args...);
+    }
+}
+
 // ----------------------------------------------- reference-layout FFTs
 
 template <typename T, int E, int D>
@@ -215,8 +247,8 @@ int launch_col_E(s2d_ctx *c, cpx<T> *data, int normalize)
     allow_lds(kern, lds);
     const int blocks = (c->g.cISize + B - 1) / B;
     const double norm = (double)c->g.nxG * (double)c->g.nyG;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(B * p.T), lds, c->stream, data, c->g.cISize, p,
-                       (const cpx<T> *)c->col.tw, B, normalize, norm);
+    launch(c, kern, dim3(blocks), dim3(B * p.T), lds, data, c->g.cISize, p,
+           (const cpx<T> *)c->col.tw, B, normalize, norm);
     S2D_LAUNCHED(c, "k_col_fft");
     return S2D_OK;
 }
@@ -249,13 +281,13 @@ int launch_row_E(s2d_ctx *c, cpx<T> *data)
     if constexpr (R2C) {
         auto kern = k_row_r2c<T, E>;
         allow_lds(kern, lds);
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(B * p.T), lds, c->stream, data, c->g.cISize,
-                           c->g.nyG, p, (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B);
+        launch(c, kern, dim3(blocks), dim3(B * p.T), lds, data, c->g.cISize, c->g.nyG, p,
+               (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B);
     } else {
         auto kern = k_row_c2r<T, E>;
         allow_lds(kern, lds);
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(B * p.T), lds, c->stream, data, c->g.cISize,
-                           c->g.nyG, p, (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B);
+        launch(c, kern, dim3(blocks), dim3(B * p.T), lds, data, c->g.cISize, c->g.nyG, p,
+               (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B);
     }
     S2D_LAUNCHED(c, R2C ? "k_row_r2c" : "k_row_c2r");
     return S2D_OK;
@@ -500,23 +532,6 @@ int passB_pairs(const s2d_ctx *c)
     return B;
 }
 
-// Event pair around a fused launch while profiling is on.
-struct ProfScope {
-    s2d_ctx *c;
-    int slot = -1;
-    ProfScope(s2d_ctx *c_, int kind) : c(c_)
-    {
-        if (c->prof_on && c->prof_used < (int)c->prof_kind.size()) {
-            slot = c->prof_used++;
-            c->prof_kind[slot] = kind;
-            (void)hipEventRecord(c->prof_ev[2 * slot], c->stream);
-        }
-    }
-    ~ProfScope()
-    {
-        if (slot >= 0) (void)hipEventRecord(c->prof_ev[2 * slot + 1], c->stream);
-    }
-};
 
 template <typename T, int E>
 int launch_passA(s2d_ctx *c, const cpx<T> *src, int reset)
@@ -527,9 +542,9 @@ int launch_passA(s2d_ctx *c, const cpx<T> *src, int reset)
     const size_t lds = (size_t)3 * d.Np * sizeof(cpx<T>);
     auto kern = k_passA<T, E>;
     allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(d.K), dim3(3 * p.T), lds, c->stream, src, (cpx<T> *)c->G[0],
-                       (cpx<T> *)c->G[1], (cpx<T> *)c->G[2], d, p, (const cpx<T> *)c->col.tw,
-                       c->d_sc, reset);
+    launch(c, kern, dim3(d.K), dim3(3 * p.T), lds, src, (cpx<T> *)c->G[0],
+           (cpx<T> *)c->G[1], (cpx<T> *)c->G[2], d, p, (const cpx<T> *)c->col.tw,
+           c->d_sc, reset);
     S2D_LAUNCHED(c, "k_passA");
     return S2D_OK;
 }
@@ -545,10 +560,10 @@ int launch_passB(s2d_ctx *c, int forward, int track_max)
     auto kern = k_passB<T, E>;
     allow_lds(kern, lds);
     const int blocks = (d.Np + 2 * B - 1) / (2 * B);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(B * p.T), lds, c->stream, (const cpx<T> *)c->G[0],
-                       (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0],
-                       (cpx<T> *)c->H[1], d, p, (const cpx<T> *)c->rowf.tw, c->d_sc, B, forward,
-                       track_max);
+    launch(c, kern, dim3(blocks), dim3(B * p.T), lds, (const cpx<T> *)c->G[0],
+           (const cpx<T> *)c->G[1], (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0],
+           (cpx<T> *)c->H[1], d, p, (const cpx<T> *)c->rowf.tw, c->d_sc, B, forward,
+           track_max);
     S2D_LAUNCHED(c, "k_passB");
     return S2D_OK;
 }
@@ -562,9 +577,9 @@ int launch_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &a
     const size_t lds = (size_t)2 * d.Np * sizeof(cpx<T>);
     auto kern = k_passC<T, E>;
     allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(d.K), dim3(2 * p.T), lds, c->stream, (const cpx<T> *)c->H[0],
-                       (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn, Wst, Wout, d, p,
-                       (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a);
+    launch(c, kern, dim3(d.K), dim3(2 * p.T), lds, (const cpx<T> *)c->H[0],
+           (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn, Wst, Wout, d, p,
+           (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a);
     S2D_LAUNCHED(c, "k_passC");
     return S2D_OK;
 }
@@ -614,8 +629,7 @@ int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
     FusedDims d = fdims(c);
     int blocks = passA_blocks<T, NP, LPB>(d.K, c->nruns);
     blocks = (blocks + 7) / 8 * 8; // a multiple of 8 for the XCD-major order
-    hipLaunchKernelGGL((k2_passA<T, NP>), dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0,
-                       c->stream, src, (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2],
+    launch(c, k2_passA<T, NP>, dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0, src, (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2],
                        d, (const cpx<T> *)c->col.tw, c->d_sc, reset, batch_of(c));
     S2D_LAUNCHED(c, "k2_passA");
     return S2D_OK;
@@ -628,8 +642,8 @@ int launch2_passB(s2d_ctx *c, int forward, int track_max)
     FusedDims d = fdims(c);
     int blocks = passB_blocks<T, NP, LPB>(c->nruns);
     blocks = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((k2_passB<T, NP>), dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0,
-                       c->stream, (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1],
+    launch(c, k2_passB<T, NP>, dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0,
+           (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1],
                        (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d,
                        (const cpx<T> *)c->rowf.tw, c->d_sc, forward, track_max, batch_of(c));
     S2D_LAUNCHED(c, "k2_passB");
@@ -642,8 +656,7 @@ int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
     const int blocks = (d.K * c->nruns + Cf::GPB - 1) / Cf::GPB;
-    hipLaunchKernelGGL((k2_passC<T, NP>), dim3(blocks), dim3(2 * Cf::BLK), 0, c->stream,
-                       (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
+    launch(c, k2_passC<T, NP>, dim3(blocks), dim3(2 * Cf::BLK), 0, (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
                        Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a,
                        batch_of(c));
     S2D_LAUNCHED(c, "k2_passC");
