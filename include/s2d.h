@@ -173,9 +173,28 @@ int s2d_rk3_steps(s2d_ctx *ctx, int nsteps, double nu, double dt, double cfl);
 int s2d_cfl_dt(s2d_ctx *ctx, double cfl, double *dt);
 /* FP64 device reductions of the current state. */
 int s2d_diagnostics(s2d_ctx *ctx, s2d_diag *out);
+/* ComputeSpectrum, src/Analysis.F90:267-460, on the device in FP64: shell
+ * energies ek[nbins] of the current state (shell b: floor(|k| + 1/2) = b,
+ * weights 1/2 at kx = 0 and kx = Np/2), the padded-spectrum mode counts
+ * cnt[nbins] (weights 1 / 2), TKE = sum ek, eps = 2 nu sum b^2 ek and the
+ * enstrophy dissipation 2 nu sum b^4 ek. nbins = s2d_spectrum_bins(grid).
+ * Any output pointer may be NULL. */
+int s2d_spectrum_bins(const s2d_grid *g);
+int s2d_spectrum(s2d_ctx *ctx, double nu, double *ek, int *cnt, double *tke, double *eps,
+                 double *ediss);
 /* Simulation time / step count held by the context (advanced by the step
  * calls); s2d_set_time resets them. */
 int s2d_set_time(s2d_ctx *ctx, double time, long long nadv);
+
+/* Checkpoint / restart (the neighbour of WriteRestart / ReadRestart,
+ * src/IO.F90:98-143, 306-424, which write HDF5): a little-endian binary
+ * with the state in the reference spectral layout (W and PSI, FP64) plus,
+ * for fused contexts, the compact kept-modes state, simulation time, step
+ * count and last dt. Reading it into a context of the same grid restores
+ * the state exactly (a fused context of the same precision continues
+ * bitwise). spectral2d_amd/checkpoint.py reads the format. */
+int s2d_checkpoint_write(s2d_ctx *ctx, const char *path);
+int s2d_checkpoint_read(s2d_ctx *ctx, const char *path);
 
 /* Seeded decaying-turbulence initial condition into W and PSI. Replaces
  * SetTurbulentVelocity (src/SetIC.F90:125-151), whose phases come from
@@ -233,6 +252,9 @@ int s2d_ensemble_get_field(s2d_ensemble *ens, int run, int field, double *buf);
  * step is per run). flags: 0 or S2D_ENS_GRAPH. Every run is checked for
  * NaN/Inf at the end. */
 int s2d_ensemble_steps(s2d_ensemble *ens, int nsteps, double dt, double cfl, int flags);
+/* s2d_spectrum of one run. */
+int s2d_ensemble_spectrum(s2d_ensemble *ens, int run, double nu, double *ek, int *cnt,
+                          double *tke, double *eps, double *ediss);
 /* out[nruns]: s2d_diagnostics of each run. */
 int s2d_ensemble_diagnostics(s2d_ensemble *ens, s2d_diag *out);
 /* Launch schedule of a step: S2D_ENS_UNIFIED (default), one launch per

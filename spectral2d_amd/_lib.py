@@ -44,6 +44,8 @@ EXPORTS = (
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
     "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_profile_begin",
     "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
+    "s2d_spectrum_bins", "s2d_spectrum", "s2d_ensemble_spectrum",
+    "s2d_checkpoint_write", "s2d_checkpoint_read",
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
@@ -132,6 +134,11 @@ def lib():
     L.s2d_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
     L.s2d_set_turbulence.argtypes = [vp, C.c_ulonglong, C.c_double, C.c_double]
     L.s2d_set_taylor_green.argtypes = [vp]
+    L.s2d_spectrum_bins.argtypes = [P(Grid)]
+    L.s2d_checkpoint_write.argtypes = [vp, C.c_char_p]
+    L.s2d_checkpoint_read.argtypes = [vp, C.c_char_p]
+    L.s2d_spectrum.argtypes = [vp, C.c_double, dp, ip, dp, dp, dp]
+    L.s2d_ensemble_spectrum.argtypes = [vp, C.c_int, C.c_double, dp, ip, dp, dp, dp]
     L.s2d_ensemble_last_error.argtypes = [vp]
     L.s2d_ensemble_last_error.restype = C.c_char_p
     L.s2d_ensemble_create.argtypes = [C.c_int, ip, C.c_int, P(vp)]

@@ -22,6 +22,7 @@
 #include "kernels_ensemble.hpp"
 #include "kernels_fused_v2.hpp"
 #include "kernels_ref.hpp"
+#include "kernels_spectrum.hpp"
 #include "s2d_internal.hpp"
 
 using namespace s2d;
@@ -841,6 +842,63 @@ int diag_of(s2d_ctx *c, int run, s2d_diag *out)
     return S2D_OK;
 }
 
+// ComputeSpectrum of the PSI slot (brought up to date by the caller):
+// device shell sums, host mode counts and moments in shell order.
+int spectrum_of(s2d_ctx *c, double nu, double *ek, int *cnt, double *tke, double *eps,
+                double *ediss)
+{
+    const s2d_grid &g = c->g;
+    const int nb = s2d_spectrum_bins(&g);
+    // the reduction buffer holds 6 n_red doubles (>= 3072: every shell count
+    // up to N = 2048); larger grids take a buffer of their own
+    double *d_ek = c->d_red, *own = nullptr;
+    if ((size_t)nb > 6 * (size_t)c->n_red) {
+        S2D_HIP(c, hipMalloc((void **)&own, sizeof(double) * nb));
+        d_ek = own;
+    }
+    if (c->precision == S2D_FP64)
+        hipLaunchKernelGGL(k_spectrum<double>, dim3(nb), dim3(256), 0, c->stream,
+                           F<double>(c, S2D_FIELD_PSI), g.cISize, g.nyG, g.kxMG, g.kyMG,
+                           c->xMaskBool, c->yMaskBool, d_ek);
+    else
+        hipLaunchKernelGGL(k_spectrum<float>, dim3(nb), dim3(256), 0, c->stream,
+                           F<float>(c, S2D_FIELD_PSI), g.cISize, g.nyG, g.kxMG, g.kyMG,
+                           c->xMaskBool, c->yMaskBool, d_ek);
+    const hipError_t el = hipGetLastError();
+    std::vector<double> h((size_t)nb);
+    const hipError_t ec =
+        hipMemcpyAsync(h.data(), d_ek, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (own) (void)hipFree(own);
+    if (el != hipSuccess) return fail(c, S2D_ERR_HIP, "launch of k_spectrum failed");
+    if (ec != hipSuccess || es != hipSuccess)
+        return fail(c, S2D_ERR_HIP, "read-back of the spectrum failed");
+    // mode counts of the padded spectrum, weights 1 / 2 (src/Analysis.F90:389-395)
+    if (cnt) {
+        for (int b = 0; b < nb; ++b) cnt[b] = 0;
+        for (int j = 0; j < g.nyG; ++j)
+            for (int i = 0; i < g.cISize; ++i) {
+                const long long m = (long long)c->kx[i] * c->kx[i] + (long long)c->ky[j] * c->ky[j];
+                long long b = (long long)std::sqrt((double)m);
+                while (b * b > m) --b;
+                if (m > b * b + b) ++b; // floor(sqrt(m) + 1/2)
+                cnt[b] += (c->kx[i] == 0 || c->kx[i] == g.kxMG) ? 1 : 2;
+            }
+    }
+    double t = 0.0, e = 0.0, z = 0.0; // src/Analysis.F90:439-448 (dk = 1)
+    for (int b = 0; b < nb; ++b) {
+        const double kb = (double)b;
+        t = t + h[b];
+        e = e + kb * kb * h[b];
+        z = z + kb * kb * kb * kb * h[b];
+        if (ek) ek[b] = h[b];
+    }
+    if (tke) *tke = t;
+    if (eps) *eps = 2.0 * nu * e;
+    if (ediss) *ediss = 2.0 * nu * z;
+    return S2D_OK;
+}
+
 int check_ctx(const s2d_ctx *c)
 {
     if (!c) return fail(nullptr, S2D_ERR_INVALID_ARG, "null context");
@@ -1459,6 +1517,22 @@ int s2d_diagnostics(s2d_ctx *c, s2d_diag *out)
     return diag_of(c, 0, out);
 }
 
+int s2d_spectrum_bins(const s2d_grid *g)
+{
+    if (!g) return -1;
+    const double kMax = std::sqrt((double)g->kxMG * g->kxMG + (double)g->kyMG * g->kyMG);
+    return (int)(kMax + 1.0) + 1; // src/Analysis.F90:326-335
+}
+
+int s2d_spectrum(s2d_ctx *c, double nu, double *ek, int *cnt, double *tke, double *eps,
+                 double *ediss)
+{
+    S2D_TRY(check_ctx(c));
+    S2D_TRY(set_device(c));
+    S2D_TRY(ensure_ref(c));
+    return spectrum_of(c, nu, ek, cnt, tke, eps, ediss);
+}
+
 int s2d_set_time(s2d_ctx *c, double time, long long nadv)
 {
     S2D_TRY(check_ctx(c));
@@ -1554,4 +1628,5 @@ int s2d_profile_end(s2d_ctx *c, double *ms, long long *count)
 
 } // extern "C"
 
+#include "s2d_checkpoint.inc"
 #include "s2d_ensemble.inc"

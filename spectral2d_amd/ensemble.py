@@ -106,6 +106,17 @@ class Ensemble:
         self._c(L.lib().s2d_ensemble_set_schedule(
             self._h, L.ENS_PER_CLASS if per_class else L.ENS_UNIFIED))
 
+    def spectrum(self, run: int, nu: float = 0.0):
+        """Shell spectrum of one run: (ek, cnt, tke, eps, enstrophy_dissipation)."""
+        nb = L.lib().s2d_spectrum_bins(C.byref(self.grids[run]))
+        ek = np.zeros(nb, np.float64)
+        cnt = np.zeros(nb, np.int32)
+        t, e, z = C.c_double(), C.c_double(), C.c_double()
+        self._c(L.lib().s2d_ensemble_spectrum(self._h, int(run), float(nu), self._d(ek),
+                                              cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                              C.byref(t), C.byref(e), C.byref(z)))
+        return ek, cnt, t.value, e.value, z.value
+
     def diagnostics(self) -> list:
         d = (L.Diag * self.nruns)()
         self._c(L.lib().s2d_ensemble_diagnostics(self._h, d))

@@ -189,6 +189,26 @@ class Spectral2D:
         self._c(L.lib().s2d_diagnostics(self._h, C.byref(d)))
         return {k: getattr(d, k) for k, _ in L.Diag._fields_}
 
+    def ComputeSpectrum(self, nu: float = 0.0):
+        """Shell spectrum of the current state (src/Analysis.F90:267-460):
+        (ek, cnt, tke, eps, enstrophy_dissipation), FP64 on the device."""
+        nb = L.lib().s2d_spectrum_bins(C.byref(self.g))
+        ek = np.zeros(nb, np.float64)
+        cnt = np.zeros(nb, np.int32)
+        t, e, z = C.c_double(), C.c_double(), C.c_double()
+        self._c(L.lib().s2d_spectrum(self._h, float(nu), _dp(ek),
+                                     cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                     C.byref(t), C.byref(e), C.byref(z)))
+        return ek, cnt, t.value, e.value, z.value
+
+    def WriteRestart(self, path: str):
+        """Checkpoint of the state, time and step count (s2d_checkpoint_write)."""
+        self._c(L.lib().s2d_checkpoint_write(self._h, str(path).encode()))
+
+    def ReadRestart(self, path: str):
+        """Restore a checkpoint written for this grid (s2d_checkpoint_read)."""
+        self._c(L.lib().s2d_checkpoint_read(self._h, str(path).encode()))
+
     def SetTurbulence(self, seed: int = 1, k0: float = 10.0, ke: float = 0.5):
         """Seeded decaying-turbulence IC into w and psi."""
         self._c(L.lib().s2d_set_turbulence(self._h, int(seed), float(k0), float(ke)))

@@ -413,3 +413,22 @@ def test_cfl_dt_definition():
     o.compute_velocity(u, v, psi, trans=True)
     m = np.max(np.abs(o.rview(u)[:, : o.nxG]) + np.abs(o.rview(v)[:, : o.nxG]))
     assert dt == 0.5 * (2 * np.pi / 32) / m
+
+
+def test_shell_index_integer_form_matches_reference_edges():
+    """The device spectrum bins mode m = kx^2 + ky^2 into shell
+    floor(sqrt(m) + 1/2) by integer bounds (b^2 - b + 1 <= m <= b^2 + b);
+    the reference searches edges 0, 0.5, 1.5, ..., kIntMax - 0.5,
+    kIntMax + 0.1 (src/Analysis.F90:358-362, :475-513). Equal for every m
+    of a 3072^2 padded grid."""
+    kxMG = kyMG = 1536
+    nb = oracle.spectrum_bins(kxMG, kyMG)
+    kint = nb - 1
+    edges = np.concatenate([[0.0], 0.5 + np.arange(kint, dtype=np.float64), [kint + 0.1]])
+    edges = edges[: nb + 1]
+    m = np.arange(0, kxMG * kxMG + kyMG * kyMG + 1, dtype=np.int64)
+    ref = np.searchsorted(edges, np.sqrt(m.astype(np.float64)), side="right") - 1
+    b = np.floor(np.sqrt(m.astype(np.float64))).astype(np.int64)
+    b -= (b * b > m)
+    b += (m > b * b + b)
+    assert np.array_equal(b, ref)
