@@ -186,6 +186,24 @@ int s2d_spectrum(s2d_ctx *ctx, double nu, double *ek, int *cnt, double *tke, dou
  * calls); s2d_set_time resets them. */
 int s2d_set_time(s2d_ctx *ctx, double time, long long nadv);
 
+/* ---- filtered DNS and closure coupling (no reference counterpart) ----
+ * s2d_restrict: dst slot = the modes kx^2 + ky^2 <= kcut^2 of the src slot
+ * (another context's grid; same device), zero elsewhere. Both are in the
+ * reference spectral layout with normalised coefficients, which carry over
+ * unchanged; where one grid resolves +k and -k and the other has a single
+ * Nyquist sample, the Nyquist sample holds their sum. This is the sharp
+ * spectral cutoff filter and the sampling onto a coarser grid. */
+int s2d_restrict(s2d_ctx *src, int src_field, s2d_ctx *dst, int dst_field, double kcut);
+/* slot y = a * slot x + b * slot y */
+int s2d_axpby(s2d_ctx *ctx, double a, int x, double b, int y);
+/* RHS forcing of the fused step: from now on every stage computes
+ * dW = -N - nu k^2 w - F, with F the kept modes of slot f at the time of
+ * the call (frozen until the next s2d_set_forcing). The closure term of a
+ * coarse run enters as F = tau_hat. s2d_clear_forcing turns it off.
+ * Single-run contexts with the compiled fused kernels only. */
+int s2d_set_forcing(s2d_ctx *ctx, int f);
+int s2d_clear_forcing(s2d_ctx *ctx);
+
 /* Checkpoint / restart (the neighbour of WriteRestart / ReadRestart,
  * src/IO.F90:98-143, 306-424, which write HDF5): a little-endian binary
  * with the state in the reference spectral layout (W and PSI, FP64) plus,

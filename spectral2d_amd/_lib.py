@@ -45,7 +45,8 @@ EXPORTS = (
     "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_profile_begin",
     "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
     "s2d_spectrum_bins", "s2d_spectrum", "s2d_ensemble_spectrum",
-    "s2d_checkpoint_write", "s2d_checkpoint_read",
+    "s2d_checkpoint_write", "s2d_checkpoint_read", "s2d_restrict", "s2d_axpby",
+    "s2d_set_forcing", "s2d_clear_forcing",
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
@@ -135,6 +136,10 @@ def lib():
     L.s2d_set_turbulence.argtypes = [vp, C.c_ulonglong, C.c_double, C.c_double]
     L.s2d_set_taylor_green.argtypes = [vp]
     L.s2d_spectrum_bins.argtypes = [P(Grid)]
+    L.s2d_restrict.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double]
+    L.s2d_axpby.argtypes = [vp, C.c_double, C.c_int, C.c_double, C.c_int]
+    L.s2d_set_forcing.argtypes = [vp, C.c_int]
+    L.s2d_clear_forcing.argtypes = [vp]
     L.s2d_checkpoint_write.argtypes = [vp, C.c_char_p]
     L.s2d_checkpoint_read.argtypes = [vp, C.c_char_p]
     L.s2d_spectrum.argtypes = [vp, C.c_double, dp, ip, dp, dp, dp]

@@ -148,6 +148,8 @@ struct Batch {
     long long sG; // each G field, elements per run
     long long sH; // each H field, elements per run
     const double *nu; // per-run viscosity, or null: StageArgs::nu for all
+    const void *forcing; // compact-layout RHS forcing F (stride sW), or null:
+                         // dW = -N - nu k^2 w - F in every stage
 };
 
 // The kernels are written as bodies over (block index blk of nblk blocks,
@@ -408,8 +410,8 @@ template <typename T, int NP, int HALF>
 __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E],
                                              const cpx<T> *other, int t, int kx, bool active,
                                              const cpx<T> *Wn, const cpx<T> *Wst, cpx<T> *Wout,
-                                             size_t col, const StageArgs &a, double dtd,
-                                             double nu)
+                                             const cpx<T> *Fc, size_t col, const StageArgs &a,
+                                             double dtd, double nu)
 {
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
@@ -419,7 +421,7 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
     const T inv_norm = (T)(1.0 / a.norm); // multiplications instead of the
                                           // reference's divisions: <= 1 ulp
     // loads first (unconditional, in bounds), so that they overlap
-    cpx<T> ws[E / 2], w0[E / 2];
+    cpx<T> ws[E / 2], w0[E / 2], fv[E / 2];
     int c[E / 2];
     bool row[E / 2];
     static_for<0, E / 2>([&](auto iI) {
@@ -430,6 +432,12 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
             w0[i] = Wn[col + c[i]];
         }
     });
+    if (Fc) { // uniform
+        static_for<0, E / 2>([&](auto iI) {
+            constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
+            if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) fv[i] = Fc[col + c[i]];
+        });
+    }
     static_for<0, E / 2>([&](auto iI) {
         constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
         if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
@@ -447,8 +455,8 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
             const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
             const T sn = (T)nu * (qx * qx + qy * qy);
             const cpx<T> wsv = ws[i], w0v = w0[i];
-            const cpx<T> dW =
-                mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
+            cpx<T> dW = mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
+            if (Fc) dW = mk<T>(dW.re - fv[i].re, dW.im - fv[i].im); // closure / forcing
             cpx<T> out;
             if (a.stage == 1) {
                 out = mk<T>(w0v.re + dt * dW.re, w0v.im + dt * dW.im);
@@ -502,10 +510,11 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
     const double dtd = stage_dt(a, sc + run);
     const double nu = bt.nu ? bt.nu[run] : a.nu;
     const size_t col = run * bt.sW + (size_t)kx * d.KP;
+    const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
     if (half == 0) // wave-uniform
-        passC_finish<T, NP, 0>(v, other, t, kx, active, Wn, Wst, Wout, col, a, dtd, nu);
+        passC_finish<T, NP, 0>(v, other, t, kx, active, Wn, Wst, Wout, Fc, col, a, dtd, nu);
     else
-        passC_finish<T, NP, 1>(v, other, t, kx, active, Wn, Wst, Wout, col, a, dtd, nu);
+        passC_finish<T, NP, 1>(v, other, t, kx, active, Wn, Wst, Wout, Fc, col, a, dtd, nu);
     if (a.stage == 3 && active && kx == 0 && half == 0 && t == 0) {
         sc[run].dt = dtd;
         sc[run].time = sc[run].time + dtd;

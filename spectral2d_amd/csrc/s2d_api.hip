@@ -23,6 +23,7 @@
 #include "kernels_fused_v2.hpp"
 #include "kernels_ref.hpp"
 #include "kernels_spectrum.hpp"
+#include "kernels_closure.hpp"
 #include "s2d_internal.hpp"
 
 using namespace s2d;
@@ -620,6 +621,7 @@ Batch batch_of(const s2d_ctx *c)
     b.sG = c->sG;
     b.sH = c->sH;
     b.nu = c->member ? c->d_nu : nullptr;
+    b.forcing = c->forcing_on ? c->forcing : nullptr;
     return b;
 }
 
@@ -1218,7 +1220,7 @@ int s2d_destroy(s2d_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     for (int f = 0; f < S2D_NFIELDS; ++f) (void)hipFree(c->field[f]);
-    void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->d_nu, c->col.tw,
+    void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->d_nu, c->forcing, c->col.tw,
                   c->rowh.tw, c->rowf.tw, c->Wn, c->Ws, c->G[0], c->G[1], c->G[2], c->H[0],
                   c->H[1]};
     for (void *p : ps) (void)hipFree(p);
@@ -1531,6 +1533,84 @@ int s2d_spectrum(s2d_ctx *c, double nu, double *ek, int *cnt, double *tke, doubl
     S2D_TRY(set_device(c));
     S2D_TRY(ensure_ref(c));
     return spectrum_of(c, nu, ek, cnt, tke, eps, ediss);
+}
+
+int s2d_restrict(s2d_ctx *src, int sf, s2d_ctx *dst, int df, double kcut)
+{
+    S2D_TRY(check_ctx(src));
+    S2D_TRY(check_ctx(dst));
+    S2D_TRY(check_fields(src, {sf}));
+    S2D_TRY(check_fields(dst, {df}));
+    if (!(kcut >= 0.0)) return fail(dst, S2D_ERR_INVALID_ARG, "kcut must be >= 0");
+    if (src->device != dst->device)
+        return fail(dst, S2D_ERR_UNSUPPORTED, "source and destination on different devices");
+    S2D_TRY(set_device(src));
+    S2D_TRY(ensure_ref(src));
+    S2D_TRY(sync(src));
+    S2D_TRY(ensure_ref(dst));
+    const long long kc = (long long)std::floor(kcut + 1.0e-10);
+    const size_t n = nc_of(dst);
+    const int sc = src->g.cISize, sn = src->g.nyG, dc = dst->g.cISize, dn = dst->g.nyG;
+    const dim3 grid(grid_blocks(n, 256)), blk(256);
+    if (src->precision == S2D_FP64 && dst->precision == S2D_FP64)
+        hipLaunchKernelGGL((k_restrict<double, double>), grid, blk, 0, dst->stream,
+                           F<double>(src, sf), sc, sn, F<double>(dst, df), dc, dn, kc * kc);
+    else if (src->precision == S2D_FP64)
+        hipLaunchKernelGGL((k_restrict<double, float>), grid, blk, 0, dst->stream,
+                           F<double>(src, sf), sc, sn, F<float>(dst, df), dc, dn, kc * kc);
+    else if (dst->precision == S2D_FP64)
+        hipLaunchKernelGGL((k_restrict<float, double>), grid, blk, 0, dst->stream,
+                           F<float>(src, sf), sc, sn, F<double>(dst, df), dc, dn, kc * kc);
+    else
+        hipLaunchKernelGGL((k_restrict<float, float>), grid, blk, 0, dst->stream,
+                           F<float>(src, sf), sc, sn, F<float>(dst, df), dc, dn, kc * kc);
+    S2D_LAUNCHED(dst, "k_restrict");
+    wrote(dst, df);
+    return sync(dst);
+}
+
+int s2d_axpby(s2d_ctx *c, double a, int x, double b, int y)
+{
+    S2D_OP_PROLOGUE(c, x, y);
+    const size_t n = nc_of(c);
+    if (c->precision == S2D_FP64)
+        hipLaunchKernelGGL(k_axpby<double>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
+                           F<double>(c, x), F<double>(c, y), a, b, n);
+    else
+        hipLaunchKernelGGL(k_axpby<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
+                           F<float>(c, x), F<float>(c, y), a, b, n);
+    S2D_LAUNCHED(c, "k_axpby");
+    wrote(c, y);
+    return sync(c);
+}
+
+int s2d_set_forcing(s2d_ctx *c, int f)
+{
+    S2D_OP_PROLOGUE(c, f);
+    if (!c->fused || !c->wave_kernels || c->nruns != 1)
+        return fail(c, S2D_ERR_UNSUPPORTED,
+                    "a forcing term needs the fused step of a single-run context");
+    if (!c->forcing)
+        S2D_HIP(c, hipMalloc(&c->forcing, (size_t)c->sW * c->celt));
+    // the kept modes of slot f in the compact layout (the pack of W, applied
+    // to slot f through a scratch swap of the compact pointer)
+    void *keep = c->Wn;
+    c->Wn = c->forcing;
+    void *wslot = c->field[S2D_FIELD_W];
+    c->field[S2D_FIELD_W] = c->field[f];
+    const int r = S2D_DISPATCH_T(c, op_pack, 0);
+    c->field[S2D_FIELD_W] = wslot;
+    c->Wn = keep;
+    S2D_TRY(r);
+    c->forcing_on = true;
+    return sync(c);
+}
+
+int s2d_clear_forcing(s2d_ctx *c)
+{
+    S2D_TRY(check_ctx(c));
+    c->forcing_on = false;
+    return S2D_OK;
 }
 
 int s2d_set_time(s2d_ctx *c, double time, long long nadv)

@@ -69,6 +69,8 @@ struct s2d_ctx {
     long long sW = 0, sG = 0, sH = 0; // elements per run
     double *d_nu = nullptr;           // per-run viscosity (ensemble size classes)
     bool member = false;              // a size class of an ensemble
+    void *forcing = nullptr;          // compact-layout RHS forcing (s2d_set_forcing)
+    bool forcing_on = false;
 
     // fused compact state
     bool fused = false;

@@ -201,6 +201,22 @@ class Spectral2D:
                                      C.byref(t), C.byref(e), C.byref(z)))
         return ek, cnt, t.value, e.value, z.value
 
+    def restrict_to(self, f, dst: "Spectral2D", df, kcut: float):
+        """dst's slot df = the modes |k| <= kcut of this context's slot f
+        (sharp spectral cutoff, sampled on dst's grid)."""
+        check(L.lib().s2d_restrict(self._h, _fid(f), dst._h, _fid(df), float(kcut)), dst._h)
+
+    def axpby(self, a: float, x, b: float, y):
+        """slot y = a * slot x + b * slot y."""
+        self._c(L.lib().s2d_axpby(self._h, float(a), _fid(x), float(b), _fid(y)))
+
+    def set_forcing(self, f):
+        """Subtract the kept modes of slot f from every stage's RHS."""
+        self._c(L.lib().s2d_set_forcing(self._h, _fid(f)))
+
+    def clear_forcing(self):
+        self._c(L.lib().s2d_clear_forcing(self._h))
+
     def WriteRestart(self, path: str):
         """Checkpoint of the state, time and step count (s2d_checkpoint_write)."""
         self._c(L.lib().s2d_checkpoint_write(self._h, str(path).encode()))
