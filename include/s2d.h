@@ -287,6 +287,42 @@ int s2d_ensemble_set_schedule(s2d_ensemble *ens, int schedule);
 int s2d_ensemble_profile_begin(s2d_ensemble *ens, int max_launches);
 int s2d_ensemble_profile_end(s2d_ensemble *ens, double *ms, long long *count);
 
+/* ---- learned subgrid closure: a periodic 3x3 CNN (no reference
+ * counterpart) ----
+ * Six 3x3 convolutions with periodic padding: 2 -> 64 -> 64 -> 64 -> 64 ->
+ * 64 -> 1 channels, GELU after all but the last; inputs (omega, psi), output
+ * tau, all on an H x W grid (W a multiple of 128, H of 8). BF16 weights and
+ * activations, FP32 accumulation: the 64 -> 64 layers forward, backward-data
+ * and backward-weight on the matrix cores (MFMA). The parameter vector
+ * (s2d_cnn_param_count() FP32 values) is W0 [9][2][64] (tap, ci, co), b0
+ * [64]; for l = 1..4 Wl [9][64][64] (tap, co, ci), bl [64]; W5 [9][64], b5;
+ * tap t = 3 dy + dx is the offset (dy - 1, dx - 1). The inputs are scaled
+ * (in0, in1) before the first layer; the output is out * (network value);
+ * the training loss is the mean of (network value - target / out)^2.
+ * Data pointers are device pointers to FP32 planes: sample n's plane at
+ * ptr + n * stride, row pitch ld floats. Calls are synchronous. */
+typedef struct s2d_cnn s2d_cnn;
+const char *s2d_cnn_last_error(const s2d_cnn *net);
+long long s2d_cnn_param_count(void);
+int s2d_cnn_create(int H, int W, int batch_max, s2d_cnn **out);
+int s2d_cnn_destroy(s2d_cnn *net);
+/* host FP32 parameter vectors; set_params also resets the Adam state */
+int s2d_cnn_set_params(s2d_cnn *net, const float *params);
+int s2d_cnn_get_params(s2d_cnn *net, float *params);
+int s2d_cnn_set_scales(s2d_cnn *net, float in0, float in1, float out);
+/* tau (= out * network value) of B samples */
+int s2d_cnn_forward(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
+                    int ld, float *tau, long long tau_stride, int tau_ld);
+/* loss and its gradient (host FP32 vector, may be NULL) at the current
+ * parameters, for B samples against target planes */
+int s2d_cnn_gradient(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
+                     int ld, const float *target, long long target_stride, int target_ld,
+                     double *loss, float *grad);
+/* gradient, then one Adam step (beta 0.9 / 0.999, eps 1e-8) of rate lr */
+int s2d_cnn_train_step(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
+                       int ld, const float *target, long long target_stride, int target_ld,
+                       float lr, double *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,9 @@ EXPORTS = (
     "s2d_spectrum_bins", "s2d_spectrum", "s2d_ensemble_spectrum",
     "s2d_checkpoint_write", "s2d_checkpoint_read", "s2d_restrict", "s2d_axpby",
     "s2d_set_forcing", "s2d_clear_forcing",
+    "s2d_cnn_last_error", "s2d_cnn_param_count", "s2d_cnn_create", "s2d_cnn_destroy",
+    "s2d_cnn_set_params", "s2d_cnn_get_params", "s2d_cnn_set_scales", "s2d_cnn_forward",
+    "s2d_cnn_gradient", "s2d_cnn_train_step",
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
@@ -159,8 +162,23 @@ def lib():
     L.s2d_ensemble_set_schedule.argtypes = [vp, C.c_int]
     L.s2d_ensemble_profile_begin.argtypes = [vp, C.c_int]
     L.s2d_ensemble_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
+    fp, ll = P(C.c_float), C.c_longlong
+    L.s2d_cnn_last_error.argtypes = [vp]
+    L.s2d_cnn_last_error.restype = C.c_char_p
+    L.s2d_cnn_param_count.argtypes = []
+    L.s2d_cnn_create.argtypes = [C.c_int, C.c_int, C.c_int, P(vp)]
+    L.s2d_cnn_destroy.argtypes = [vp]
+    L.s2d_cnn_set_params.argtypes = [vp, fp]
+    L.s2d_cnn_get_params.argtypes = [vp, fp]
+    L.s2d_cnn_set_scales.argtypes = [vp, C.c_float, C.c_float, C.c_float]
+    L.s2d_cnn_forward.argtypes = [vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int]
+    L.s2d_cnn_gradient.argtypes = [vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int, dp, fp]
+    L.s2d_cnn_train_step.argtypes = [vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int,
+                                     C.c_float, dp]
     for name in EXPORTS:
-        if name not in ("s2d_last_error", "s2d_ensemble_last_error"):
+        if name == "s2d_cnn_param_count":
+            L.s2d_cnn_param_count.restype = C.c_longlong
+        elif name not in ("s2d_last_error", "s2d_ensemble_last_error", "s2d_cnn_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -169,6 +187,12 @@ def lib():
 def check(rc: int, ctx=None):
     if rc != S2D_OK:
         msg = lib().s2d_last_error(ctx)
+        raise S2DError(rc, msg.decode() if msg else "")
+
+
+def check_cnn(rc: int, net=None):
+    if rc != S2D_OK:
+        msg = lib().s2d_cnn_last_error(net)
         raise S2DError(rc, msg.decode() if msg else "")
 
 

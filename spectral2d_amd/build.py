@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libs2d.so")
-SOURCES = [os.path.join(CSRC, "s2d_api.hip")]
+SOURCES = [os.path.join(CSRC, "s2d_api.hip"), os.path.join(CSRC, "cnn.hip")]
 ARCH = os.environ.get("S2D_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

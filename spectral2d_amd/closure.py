@@ -1,0 +1,158 @@
+"""Filtered DNS data and the learned subgrid closure (no reference
+counterpart; the reference solves one resolution at a time).
+
+Pieces, all computing through libs2d:
+
+- :class:`FilteredSampler` -- from a fine run (N_f, 3/2 rule): the sharp
+  spectral cutoff |k| <= N_c/2 of omega and psi and the subgrid flux
+  tau = filter(u.grad omega) - u_bar.grad omega_bar, where u_bar.grad
+  omega_bar is the coarse solver's own dealiased operator (an N_c context
+  with the 3/2 rule), all sampled on the N_c x N_c grid (a context without
+  padding does the transforms).
+- :class:`ClosedRun` -- a coarse run whose every RK3 stage subtracts
+  tau_hat = FFT(CNN(omega_bar, psi_bar)), evaluated once per step at the
+  step's start (s2d_set_forcing).
+- :func:`enstrophy_spectrum` -- Z(k) = k^2 E(k) with the reference's shells
+  (src/Analysis.F90:358-400).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from ._lib import NO_DEALIASING
+from .cnn import ClosureNet
+from .spectral import FP32, Spectral2D
+
+
+def enstrophy_spectrum(sim, nu: float = 0.0):
+    ek = sim.ComputeSpectrum(nu)[0]
+    k = np.arange(ek.size, dtype=np.float64)
+    return k * k * ek
+
+
+class FilteredSampler:
+    """Filtered snapshots of a fine context on an Nc x Nc grid."""
+
+    def __init__(self, fine: Spectral2D, Nc: int):
+        self.fine = fine
+        self.Nc = Nc
+        self.kc = Nc // 2
+        self.coarse = Spectral2D(Nc, precision=fine.precision)
+        self.grid = Spectral2D(Nc, dealias=NO_DEALIASING, precision=fine.precision)
+
+    def close(self):
+        self.coarse.close()
+        self.grid.close()
+
+    def filtered_state(self, dst: Spectral2D):
+        """dst's W slot = the filtered vorticity (dst: an Nc 3/2-rule context)."""
+        self.fine.restrict_to("w", dst, "w", self.kc)
+
+    def snapshot(self):
+        """(w_bar, psi_bar, tau) as float32 arrays (Nc, Nc), physical."""
+        f, c, g, kc = self.fine, self.coarse, self.grid, self.kc
+        f.copy_field("w", "tmp")
+        f.ComputePsi("tmp", "psiS")
+        f.ComputeRHS(0.0, "u", "v", "tmp", "psiS", "dW")   # -N_fine
+        f.restrict_to("w", c, "w", kc)
+        f.restrict_to("dW", c, "tmp", kc)                  # -filter(N_fine)
+        c.ComputePsi("w", "psi")
+        c.copy_field("w", "wS")
+        c.ComputeRHS(0.0, "u", "v", "wS", "psi", "dW")     # -N_coarse(w_bar)
+        c.axpby(-1.0, "tmp", 1.0, "dW")                    # tau_hat
+        c.restrict_to("w", g, "w", kc)
+        c.restrict_to("psi", g, "psi", kc)
+        c.restrict_to("dW", g, "tmp", kc)
+        out = []
+        for s in ("w", "psi", "tmp"):
+            g.TransformC2R(s)
+            out.append(g.rview(g.get_field(s))[:, : self.Nc].astype(np.float32))
+        return tuple(out)
+
+
+class ClosedRun:
+    """A coarse (Nc, 3/2 rule) run with the CNN closure term."""
+
+    def __init__(self, Nc: int, net: ClosureNet | None, precision: int = FP32):
+        self.Nc = Nc
+        self.kc = Nc // 2
+        self.net = net
+        self.sim = Spectral2D(Nc, precision=precision)
+        self.grid = Spectral2D(Nc, dealias=NO_DEALIASING, precision=precision)
+        if net is not None and precision != FP32:
+            raise ValueError("the closure network reads FP32 planes")
+
+    def close(self):
+        self.sim.close()
+        self.grid.close()
+
+    def closure_forcing(self):
+        s, g, kc = self.sim, self.grid, self.kc
+        s.restrict_to("w", g, "w", kc)
+        s.restrict_to("psi", g, "psi", kc)
+        g.TransformC2R("w")
+        g.TransformC2R("psi")
+        ld = g.rISize
+        self.net.forward(1, g.device_ptr("w"), g.device_ptr("psi"), 0, ld,
+                         g.device_ptr("tmp"), 0, ld)
+        g.TransformR2C("tmp")
+        g.restrict_to("tmp", s, "tmp", kc)
+        s.set_forcing("tmp")
+
+    def advance_to(self, t_end: float, nu: float, cfl: float = 0.5, max_steps: int = 10 ** 7):
+        """Steps (closure refreshed every step) until the time reaches t_end;
+        the last step is clipped to land on t_end."""
+        s = self.sim
+        n = 0
+        while n < max_steps:
+            t = s.diagnostics()["time"]
+            if t >= t_end - 1e-12:
+                break
+            dt = s.ComputeTimeStep(cfl)
+            if t + dt > t_end:
+                dt = t_end - t
+            if self.net is not None:
+                self.closure_forcing()
+            s.rk3_steps(1, nu, dt, 0.0)
+            n += 1
+        return n
+
+
+def advance_to(sim: Spectral2D, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 20):
+    """Fused CFL steps in chunks, then fixed steps that land on t_end."""
+    n = 0
+    while True:
+        t = sim.diagnostics()["time"]
+        if t >= t_end - 1e-12:
+            return n
+        dt = sim.ComputeTimeStep(cfl)
+        if t + chunk * 1.2 * dt < t_end:
+            sim.rk3_steps(chunk, nu, 0.0, cfl)
+            n += chunk
+        else:
+            sim.rk3_steps(1, nu, min(dt, t_end - t), 0.0)
+            n += 1
+
+
+def spectrum_error(z, z_ref, kmax: int):
+    """Relative L2 distance of enstrophy spectra over shells 1..kmax."""
+    a, b = np.asarray(z)[1:kmax + 1], np.asarray(z_ref)[1:kmax + 1]
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+class Timer:
+    def __init__(self):
+        self.t = 0.0
+
+    def __enter__(self):
+        self.t0 = time.perf_counter()
+        return self
+
+    def __exit__(self, *a):
+        self.t += time.perf_counter() - self.t0
+
+
+__all__ = ["FilteredSampler", "ClosedRun", "enstrophy_spectrum", "advance_to", "spectrum_error",
+           "Timer"]

@@ -124,6 +124,14 @@ class Spectral2D:
     def copy_field(self, src, dst):
         self._c(L.lib().s2d_copy_field(self._h, _fid(src), _fid(dst)))
 
+    def device_ptr(self, f) -> int:
+        """Device address of slot f (brought up to date with the state);
+        FP64 or FP32 values per the context's precision, in the reference
+        layout (rows of rISize reals / cISize complex)."""
+        ptr = C.c_void_p()
+        self._c(L.lib().s2d_field_device_ptr(self._h, _fid(f), C.byref(ptr)))
+        return int(ptr.value)
+
     def sync_fields(self):
         self._c(L.lib().s2d_sync_fields(self._h))
 
