@@ -185,6 +185,8 @@ int s2d_spectrum(s2d_ctx *ctx, double nu, double *ek, int *cnt, double *tke, dou
 /* Simulation time / step count held by the context (advanced by the step
  * calls); s2d_set_time resets them. */
 int s2d_set_time(s2d_ctx *ctx, double time, long long nadv);
+/* time, step count and dt of the last step (any pointer may be NULL) */
+int s2d_get_time(s2d_ctx *ctx, double *time, long long *nadv, double *dt);
 
 /* ---- filtered DNS and closure coupling (no reference counterpart) ----
  * s2d_restrict: dst slot = the modes kx^2 + ky^2 <= kcut^2 of the src slot
@@ -203,6 +205,20 @@ int s2d_axpby(s2d_ctx *ctx, double a, int x, double b, int y);
  * Single-run contexts with the compiled fused kernels only. */
 int s2d_set_forcing(s2d_ctx *ctx, int f);
 int s2d_clear_forcing(s2d_ctx *ctx);
+/* The closure forcing of a coarse run in one call: the state's w and psi
+ * sampled on `grid` (a context of the same N without padding, FP32) by the
+ * sharp cutoff |k| <= N/2, tau = net(w, psi) there (s2d_cnn_*, declared
+ * below), its transform restricted back, then s2d_set_forcing. One stream,
+ * one wait at the end. Overwrites grid's W, PSI and TMP slots and ctx's TMP
+ * slot. */
+struct s2d_cnn;
+int s2d_closure_forcing(s2d_ctx *ctx, s2d_ctx *grid, struct s2d_cnn *net);
+/* nsteps fused CFL steps (cfl > 0) of a closure run, the forcing refreshed
+ * at every `every`-th step: one step (with or without the refresh) is
+ * captured once into a hipGraph and replayed (re-captured when the grid,
+ * network, nu or cfl change). Checked for NaN/Inf at the end. */
+int s2d_closure_steps(s2d_ctx *ctx, s2d_ctx *grid, struct s2d_cnn *net, int nsteps, double nu,
+                      double cfl, int every);
 
 /* Checkpoint / restart (the neighbour of WriteRestart / ReadRestart,
  * src/IO.F90:98-143, 306-424, which write HDF5): a little-endian binary
@@ -310,9 +326,16 @@ int s2d_cnn_destroy(s2d_cnn *net);
 int s2d_cnn_set_params(s2d_cnn *net, const float *params);
 int s2d_cnn_get_params(s2d_cnn *net, float *params);
 int s2d_cnn_set_scales(s2d_cnn *net, float in0, float in1, float out);
+/* clip != 0: s2d_cnn_forward keeps tau only where omega * tau > 0 (the
+ * closure term -tau then only removes enstrophy, pointwise); 0: raw */
+int s2d_cnn_set_clip(s2d_cnn *net, int clip);
 /* tau (= out * network value) of B samples */
 int s2d_cnn_forward(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
                     int ld, float *tau, long long tau_stride, int tau_ld);
+/* the same on a caller's HIP stream (a hipStream_t), without waiting */
+int s2d_cnn_forward_on(s2d_cnn *net, void *stream, int B, const float *w, const float *psi,
+                       long long stride, int ld, float *tau, long long tau_stride, int tau_ld);
+int s2d_cnn_grid(const s2d_cnn *net, int *H, int *W);
 /* loss and its gradient (host FP32 vector, may be NULL) at the current
  * parameters, for B samples against target planes */
 int s2d_cnn_gradient(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
@@ -322,6 +345,14 @@ int s2d_cnn_gradient(s2d_cnn *net, int B, const float *w, const float *psi, long
 int s2d_cnn_train_step(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
                        int ld, const float *target, long long target_stride, int target_ld,
                        float lr, double *loss);
+/* A training set held on the device: n samples of dense H x W FP32 planes
+ * (host arrays w[n][H][W], psi, tau); s2d_cnn_train_batch gathers samples
+ * idx[0..B-1] and takes one s2d_cnn_train_step on them; s2d_cnn_predict
+ * writes the network's tau of sample i to a host plane. */
+int s2d_cnn_set_dataset(s2d_cnn *net, long long n, const float *w, const float *psi,
+                        const float *tau);
+int s2d_cnn_train_batch(s2d_cnn *net, int B, const int *idx, float lr, double *loss);
+int s2d_cnn_predict(s2d_cnn *net, long long i, float *tau);
 
 #ifdef __cplusplus
 }

@@ -42,14 +42,16 @@ EXPORTS = (
     "s2d_transform_c2r", "s2d_dudx", "s2d_dudy", "s2d_truncate",
     "s2d_compute_psi", "s2d_compute_velocity", "s2d_compute_vorticity",
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
-    "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_profile_begin",
+    "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_get_time", "s2d_profile_begin",
     "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
     "s2d_spectrum_bins", "s2d_spectrum", "s2d_ensemble_spectrum",
     "s2d_checkpoint_write", "s2d_checkpoint_read", "s2d_restrict", "s2d_axpby",
     "s2d_set_forcing", "s2d_clear_forcing",
     "s2d_cnn_last_error", "s2d_cnn_param_count", "s2d_cnn_create", "s2d_cnn_destroy",
     "s2d_cnn_set_params", "s2d_cnn_get_params", "s2d_cnn_set_scales", "s2d_cnn_forward",
-    "s2d_cnn_gradient", "s2d_cnn_train_step",
+    "s2d_cnn_gradient", "s2d_cnn_train_step", "s2d_cnn_set_dataset", "s2d_cnn_train_batch",
+    "s2d_cnn_predict", "s2d_cnn_set_clip", "s2d_cnn_forward_on", "s2d_cnn_grid",
+    "s2d_closure_forcing", "s2d_closure_steps",
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
@@ -134,6 +136,7 @@ def lib():
     L.s2d_cfl_dt.argtypes = [vp, C.c_double, dp]
     L.s2d_diagnostics.argtypes = [vp, P(Diag)]
     L.s2d_set_time.argtypes = [vp, C.c_double, C.c_longlong]
+    L.s2d_get_time.argtypes = [vp, dp, P(C.c_longlong), dp]
     L.s2d_profile_begin.argtypes = [vp, C.c_int]
     L.s2d_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
     L.s2d_set_turbulence.argtypes = [vp, C.c_ulonglong, C.c_double, C.c_double]
@@ -175,6 +178,14 @@ def lib():
     L.s2d_cnn_gradient.argtypes = [vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int, dp, fp]
     L.s2d_cnn_train_step.argtypes = [vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int,
                                      C.c_float, dp]
+    L.s2d_cnn_set_dataset.argtypes = [vp, ll, fp, fp, fp]
+    L.s2d_cnn_train_batch.argtypes = [vp, C.c_int, ip, C.c_float, dp]
+    L.s2d_cnn_predict.argtypes = [vp, ll, fp]
+    L.s2d_cnn_set_clip.argtypes = [vp, C.c_int]
+    L.s2d_cnn_forward_on.argtypes = [vp, vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int]
+    L.s2d_cnn_grid.argtypes = [vp, ip, ip]
+    L.s2d_closure_forcing.argtypes = [vp, vp, vp]
+    L.s2d_closure_steps.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_int]
     for name in EXPORTS:
         if name == "s2d_cnn_param_count":
             L.s2d_cnn_param_count.restype = C.c_longlong

@@ -75,10 +75,12 @@ class FilteredSampler:
 class ClosedRun:
     """A coarse (Nc, 3/2 rule) run with the CNN closure term."""
 
-    def __init__(self, Nc: int, net: ClosureNet | None, precision: int = FP32):
+    def __init__(self, Nc: int, net: ClosureNet | None, precision: int = FP32,
+                 every: int = 1):
         self.Nc = Nc
         self.kc = Nc // 2
         self.net = net
+        self.every = every  # closure evaluations: every `every` steps (frozen between)
         self.sim = Spectral2D(Nc, precision=precision)
         self.grid = Spectral2D(Nc, dealias=NO_DEALIASING, precision=precision)
         if net is not None and precision != FP32:
@@ -89,6 +91,12 @@ class ClosedRun:
         self.grid.close()
 
     def closure_forcing(self):
+        """s2d_closure_forcing: the whole sequence in one library call."""
+        from . import _lib as L
+        L.check(L.lib().s2d_closure_forcing(self.sim._h, self.grid._h, self.net._h), self.sim._h)
+
+    def closure_forcing_steps(self):
+        """The same sequence through the per-routine entry points."""
         s, g, kc = self.sim, self.grid, self.kc
         s.restrict_to("w", g, "w", kc)
         s.restrict_to("psi", g, "psi", kc)
@@ -101,39 +109,65 @@ class ClosedRun:
         g.restrict_to("tmp", s, "tmp", kc)
         s.set_forcing("tmp")
 
-    def advance_to(self, t_end: float, nu: float, cfl: float = 0.5, max_steps: int = 10 ** 7):
-        """Steps (closure refreshed every step) until the time reaches t_end;
-        the last step is clipped to land on t_end."""
+    def steps(self, n: int, nu: float, cfl: float = 0.5):
+        """n CFL steps; with a network, the forcing is refreshed every
+        `every` steps and the steps replay a captured hipGraph
+        (s2d_closure_steps)."""
+        from . import _lib as L
+        if self.net is None:
+            self.sim.rk3_steps(n, nu, 0.0, cfl)
+        else:
+            L.check(L.lib().s2d_closure_steps(self.sim._h, self.grid._h, self.net._h, int(n),
+                                              float(nu), float(cfl), int(self.every)),
+                    self.sim._h)
+
+    def advance_to(self, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 25,
+                   max_steps: int = 20000, z_limit: float = 4.0, log=None):
+        """Chunks of CFL steps until the time reaches t_end, then fixed steps
+        that land on it. Raises RuntimeError when the enstrophy exceeds
+        z_limit times its initial value or max_steps is reached (an unstable
+        closure)."""
         s = self.sim
         n = 0
-        while n < max_steps:
-            t = s.diagnostics()["time"]
-            if t >= t_end - 1e-12:
-                break
+        t, _, dt = s.get_time()
+        if dt <= 0.0:
             dt = s.ComputeTimeStep(cfl)
-            if t + dt > t_end:
-                dt = t_end - t
-            if self.net is not None:
-                self.closure_forcing()
-            s.rk3_steps(1, nu, dt, 0.0)
-            n += 1
+        z0 = s.diagnostics()["enstrophy"]
+        while t < t_end - 1e-12:
+            if n >= max_steps:
+                raise RuntimeError(f"no arrival at t = {t_end} within {max_steps} steps (t = {t})")
+            if t + chunk * 1.25 * dt < t_end:
+                self.steps(chunk, nu, cfl)
+                n += chunk
+                z = s.diagnostics()["enstrophy"]
+                if not (z <= z_limit * z0):
+                    raise RuntimeError(f"enstrophy grew {z / z0:.2f}x by t = {t:.4f}")
+                if log and n % (4 * chunk) == 0:
+                    log(f"  coarse run t = {t:.4f}, {n} steps, Z/Z0 = {z / z0:.3f}")
+            else:
+                if self.net is not None:
+                    self.closure_forcing()
+                s.rk3_steps(1, nu, min(s.ComputeTimeStep(cfl), t_end - t), 0.0)
+                n += 1
+            t, _, dt = s.get_time()
         return n
 
 
 def advance_to(sim: Spectral2D, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 20):
     """Fused CFL steps in chunks, then fixed steps that land on t_end."""
     n = 0
-    while True:
-        t = sim.diagnostics()["time"]
-        if t >= t_end - 1e-12:
-            return n
+    t, _, dt = sim.get_time()
+    if dt <= 0.0:
         dt = sim.ComputeTimeStep(cfl)
-        if t + chunk * 1.2 * dt < t_end:
+    while t < t_end - 1e-12:
+        if t + chunk * 1.25 * dt < t_end:
             sim.rk3_steps(chunk, nu, 0.0, cfl)
             n += chunk
         else:
-            sim.rk3_steps(1, nu, min(dt, t_end - t), 0.0)
+            sim.rk3_steps(1, nu, min(sim.ComputeTimeStep(cfl), t_end - t), 0.0)
             n += 1
+        t, _, dt = sim.get_time()
+    return n
 
 
 def spectrum_error(z, z_ref, kmax: int):

@@ -1,0 +1,227 @@
+"""The learned-closure pipeline end to end, on one GPU:
+
+1. training data: seeded decaying-turbulence DNS runs (seeds 2000 + r,
+   N_f^2, FP32 with FP64 reductions, k0 = 12, CFL 0.5); every `every` steps
+   after `t_spin`, the filtered omega, psi and the subgrid flux tau on the
+   N_c^2 grid (FilteredSampler);
+2. training: the closure CNN on those samples (the library's forward,
+   backward and Adam kernels), inputs and output normalised by their
+   standard deviations;
+3. evaluation: held-out seeds 3000 + m; each DNS runs to t0, its filtered
+   state starts a plain N_c run and a closure run, and the DNS continues to
+   t1; the ensemble-mean enstrophy spectra Z(k) = k^2 E(k) of the two coarse
+   runs are compared with the filtered DNS over shells 1..N_c/2 (relative
+   L2); the GPU time of the DNS over [t0, t1] is divided by that of the
+   closure run.
+
+Writes <out>/closure_cnn.npz (parameters and scales) and <out>/metrics.json.
+Run: python -m spectral2d_amd.closure_run --out DIR [options]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(time.strftime("%H:%M:%S"), *a, flush=True)
+
+
+def make_dns(Nf, seed, k0, nu):
+    from .spectral import FP32, Spectral2D
+    s = Spectral2D(Nf, precision=FP32)
+    s.SetTurbulence(seed, k0, 0.5)
+    s.set_time(0.0, 0)
+    return s
+
+
+def gen_data(a):
+    from .closure import FilteredSampler, advance_to
+    X, T = [], []
+    for r in range(a.train_runs):
+        s = make_dns(a.nf, 2000 + r, a.k0, a.nu)
+        sm = FilteredSampler(s, a.nc)
+        advance_to(s, a.t_spin, a.nu, a.cfl)
+        n = 0
+        while s.get_time()[0] < a.t_end:
+            s.rk3_steps(a.every, a.nu, 0.0, a.cfl)
+            w, p, tau = sm.snapshot()
+            X.append(np.stack([w, p]))
+            T.append(tau)
+            n += 1
+        log(f"data run {r}: {n} samples, t = {s.get_time()[0]:.3f}")
+        sm.close()
+        s.close()
+    return np.stack(X).astype(np.float32), np.stack(T).astype(np.float32)
+
+
+def train_net(a, X, T, params0=None):
+    """Adam on the dataset held by the library; cosine learning rate."""
+    from .cnn import ClosureNet, param_count
+    n, _, H, W = X.shape
+    sw, sp, st = (float(X[:, 0].std()), float(X[:, 1].std()), float(T.std()))
+    B = min(a.batch, n)
+    net = ClosureNet(H, W, B)
+    net.set_params(params0 if params0 is not None else init_params(a.seed, param_count()))
+    net.set_scales(1.0 / sw, 1.0 / sp, st)
+    net.set_dataset(X[:, 0], X[:, 1], T)
+    rng = np.random.default_rng(a.seed)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(a.iters):
+        lr = a.lr * (0.5 * (1 + np.cos(np.pi * it / a.iters)))
+        hist.append(net.train_batch(rng.choice(n, B, replace=False), lr))
+        if it % 200 == 0 or it == a.iters - 1:
+            log(f"train it {it}: loss {np.mean(hist[-50:]):.4f} (lr {lr:.2e})")
+    el = time.perf_counter() - t0
+    return net, (sw, sp, st), hist, el
+
+
+def init_params(seed, n):
+    """He-style initialisation of the network parameter vector (layout of
+    include/s2d.h, s2d_cnn_*), small last layer."""
+    rng = np.random.default_rng(seed)
+    C = 64
+    parts = [rng.standard_normal(9 * 2 * C) * np.sqrt(2.0 / 18), np.zeros(C)]
+    for _ in range(4):
+        parts += [rng.standard_normal(9 * C * C) * np.sqrt(2.0 / (9 * C)), np.zeros(C)]
+    parts += [rng.standard_normal(9 * C) * np.sqrt(0.1 / (9 * C)), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == n
+    return p
+
+
+def evaluate(a, net):
+    from .closure import ClosedRun, FilteredSampler, Timer, advance_to, enstrophy_spectrum
+    from .closure import spectrum_error
+    kmax = a.nc // 2
+    Z = {"dns": [], "plain": [], "closure": []}
+    t_dns, t_clo, t_pln = Timer(), Timer(), Timer()
+    steps = {"dns": 0, "plain": 0, "closure": 0}
+    unstable = {}
+    for m in range(a.eval_runs):
+        s = make_dns(a.nf, 3000 + m, a.k0, a.nu)
+        advance_to(s, a.t0, a.nu, a.cfl)
+        runs = {}
+        for kind in ("plain", "closure"):
+            r = ClosedRun(a.nc, net if kind == "closure" else None, every=a.closure_every)
+            s.restrict_to("w", r.sim, "w", kmax)
+            r.sim.ComputePsi("w", "psi")
+            r.sim.set_time(s.get_time()[0], 0)
+            runs[kind] = r
+        with t_dns:
+            steps["dns"] += advance_to(s, a.t1, a.nu, a.cfl)
+        log(f"eval member {m}: DNS reached t1 ({steps['dns']} steps so far)")
+        fs = FilteredSampler(s, a.nc)
+        fs.filtered_state(fs.coarse)
+        fs.coarse.ComputePsi("w", "psi")
+        Z["dns"].append(enstrophy_spectrum(fs.coarse))
+        fs.close()
+        s.close()
+        for kind, r in runs.items():
+            try:
+                with (t_clo if kind == "closure" else t_pln):
+                    steps[kind] += r.advance_to(a.t1, a.nu, a.cfl, log=log)
+                Z[kind].append(enstrophy_spectrum(r.sim))
+            except RuntimeError as ex:
+                log(f"  {kind} run of member {m} unstable: {ex}")
+                unstable[kind] = unstable.get(kind, 0) + 1
+            r.close()
+        if Z["plain"] and Z["closure"]:
+            e_p = spectrum_error(np.mean(Z["plain"], 0), np.mean(Z["dns"], 0), kmax)
+            e_c = spectrum_error(np.mean(Z["closure"], 0), np.mean(Z["dns"], 0), kmax)
+            log(f"eval member {m}: ensemble errors so far plain {e_p:.4f} closure {e_c:.4f}")
+    if not Z["closure"]:
+        return {"members": a.eval_runs, "unstable": unstable, "steps": steps}
+    zd, zp, zc = (np.mean(Z[k], 0) for k in ("dns", "plain", "closure"))
+    return {
+        "unstable": unstable,
+        "members": a.eval_runs,
+        "t0": a.t0, "t1": a.t1,
+        "err_plain": spectrum_error(zp, zd, kmax),
+        "err_closure": spectrum_error(zc, zd, kmax),
+        "gpu_s_dns": t_dns.t, "gpu_s_closure": t_clo.t, "gpu_s_plain": t_pln.t,
+        "speedup_vs_dns": t_dns.t / max(t_clo.t, 1e-12),
+        "steps": steps,
+        "Z_dns": zd[: kmax + 1].tolist(), "Z_plain": zp[: kmax + 1].tolist(),
+        "Z_closure": zc[: kmax + 1].tolist(),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--nf", type=int, default=2048)
+    ap.add_argument("--nc", type=int, default=256)
+    ap.add_argument("--nu", type=float, default=2e-5)
+    ap.add_argument("--k0", type=float, default=12.0)
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--train-runs", type=int, default=16)
+    ap.add_argument("--t-spin", type=float, default=0.4)
+    ap.add_argument("--t-end", type=float, default=1.0)
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--eval-runs", type=int, default=8)
+    ap.add_argument("--t0", type=float, default=0.5)
+    ap.add_argument("--t1", type=float, default=1.0)
+    ap.add_argument("--clip", type=int, default=1,
+                    help="1: keep only the enstrophy-sink part of tau (omega tau > 0)")
+    ap.add_argument("--closure-every", type=int, default=1,
+                    help="closure evaluations every this many coarse steps")
+    ap.add_argument("--weights", help="skip data and training: evaluate these weights")
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    from .cnn import ClosureNet
+    metrics = {"config": vars(a)}
+    if a.weights:
+        d = np.load(a.weights)
+        net = ClosureNet(a.nc, a.nc, 1)
+        net.set_params(d["params"])
+        net.set_scales(*[float(v) for v in d["scales"]])
+    else:
+        log("generating filtered DNS data")
+        t0 = time.perf_counter()
+        X, T = gen_data(a)
+        metrics["data"] = {"samples": int(len(X)), "seconds": time.perf_counter() - t0,
+                           "std": [float(X[:, 0].std()), float(X[:, 1].std()), float(T.std())]}
+        log(f"{len(X)} samples in {metrics['data']['seconds']:.1f} s")
+        net, scales, hist, el = train_net(a, X, T)
+        # a-priori: correlation of the prediction with tau on a few samples
+        k = min(32, len(X))
+        pred = np.stack([net.predict(i) for i in range(k)])
+        cc = float(np.corrcoef(pred.ravel(), T[:k].ravel())[0, 1])
+        metrics["train"] = {"iters": a.iters, "seconds": el, "loss_first": float(np.mean(hist[:20])),
+                            "loss_last": float(np.mean(hist[-50:])), "corr_apriori": cc}
+        log(f"trained: {el:.1f} s, loss {metrics['train']['loss_first']:.4f} -> "
+            f"{metrics['train']['loss_last']:.4f}, a-priori correlation {cc:.3f}")
+        params = net.get_params()
+        np.savez(os.path.join(a.out, "closure_cnn.npz"), params=params,
+                 scales=np.array(net.scales, np.float32))
+        net.close()
+        net = ClosureNet(a.nc, a.nc, 1)
+        net.set_params(params)
+        net.set_scales(*scales)
+    net.set_clip(a.clip)
+    metrics["eval"] = evaluate(a, net)
+    e = metrics["eval"]
+    if "err_closure" in e:
+        log(f"spectrum error plain {e['err_plain']:.4f} closure {e['err_closure']:.4f}; "
+            f"DNS {e['gpu_s_dns']:.2f} s vs closure {e['gpu_s_closure']:.2f} s "
+            f"({e['speedup_vs_dns']:.1f}x); unstable {e['unstable']}")
+    else:
+        log(f"every closure run unstable: {e}")
+    with open(os.path.join(a.out, "metrics.json"), "w") as f:
+        json.dump(metrics, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

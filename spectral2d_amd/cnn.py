@@ -59,6 +59,10 @@ class ClosureNet:
         self.scales = (float(in0), float(in1), float(out))
         self._c(L.lib().s2d_cnn_set_scales(self._h, *self.scales))
 
+    def set_clip(self, clip: bool):
+        """Keep only the enstrophy-sink part of tau (omega * tau > 0)."""
+        self._c(L.lib().s2d_cnn_set_clip(self._h, int(bool(clip))))
+
     def forward(self, B, w_ptr, psi_ptr, stride, ld, tau_ptr, tau_stride, tau_ld):
         self._c(L.lib().s2d_cnn_forward(self._h, int(B), C.c_void_p(w_ptr), C.c_void_p(psi_ptr),
                                         int(stride), int(ld), C.c_void_p(tau_ptr),
@@ -80,6 +84,29 @@ class ClosureNet:
                                            C.c_void_p(tg_ptr), int(tg_stride), int(tg_ld),
                                            float(lr), C.byref(loss)))
         return loss.value
+
+
+    def set_dataset(self, w: np.ndarray, psi: np.ndarray, tau: np.ndarray):
+        """Host arrays (n, H, W) copied to the device for train_batch."""
+        arrs = [np.ascontiguousarray(x, np.float32) for x in (w, psi, tau)]
+        n = arrs[0].shape[0]
+        fp = C.POINTER(C.c_float)
+        self._c(L.lib().s2d_cnn_set_dataset(self._h, n, *[x.ctypes.data_as(fp) for x in arrs]))
+        self.ds_n = n
+
+    def train_batch(self, idx, lr: float) -> float:
+        idx = np.ascontiguousarray(idx, np.int32)
+        loss = C.c_double()
+        self._c(L.lib().s2d_cnn_train_batch(self._h, int(idx.size),
+                                            idx.ctypes.data_as(C.POINTER(C.c_int)), float(lr),
+                                            C.byref(loss)))
+        return loss.value
+
+    def predict(self, i: int) -> np.ndarray:
+        out = np.zeros((self.H, self.W), np.float32)
+        self._c(L.lib().s2d_cnn_predict(self._h, int(i),
+                                        out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
 
 
 __all__ = ["ClosureNet", "param_count"]

@@ -37,7 +37,8 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int CH = 64;       // channels of the hidden layers
 constexpr int NMID = 4;      // 64 -> 64 layers
-constexpr int TPX = 128;     // pixels per conv tile (one row segment)
+constexpr int TPX = 128;     // pixels per weight-gradient tile (one row segment)
+constexpr int CMT = 1;       // M tiles per wave of the forward / data conv (64 CMT pixels)
 constexpr int LP = CH + 8;   // LDS pitch (BF16) of a staged pixel
 constexpr int W64 = 9 * CH * CH;
 
@@ -90,61 +91,60 @@ __global__ __launch_bounds__(256) void k_conv_in(const float *__restrict__ x0,
     for (int i = threadIdx.x; i < CH; i += blockDim.x) w[9 * 2 * CH + i] = P[OFF_B0 + i];
     __syncthreads();
     const long long npx = (long long)B * H * W;
-    S2D_GRID_STRIDE(p, npx)
+    S2D_GRID_STRIDE(q, npx * 8) // (pixel, group of 8 output channels)
     {
+        const long long p = q >> 3;
+        const int c8 = 8 * (int)(q & 7);
         const int xx = (int)(p % W), yy = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
-        float in[9][2];
+        float acc[8];
+        for (int k = 0; k < 8; ++k) acc[k] = w[9 * 2 * CH + c8 + k];
         for (int t = 0; t < 9; ++t) {
             const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
             const long long o = n * sN + (long long)y2 * ld + x2;
-            in[t][0] = bfr(s0 * x0[o]);
-            in[t][1] = bfr(s1 * x1[o]);
+            const float i0 = bfr(s0 * x0[o]), i1 = bfr(s1 * x1[o]);
+            for (int k = 0; k < 8; ++k)
+                acc[k] += i0 * w[(t * 2 + 0) * CH + c8 + k] + i1 * w[(t * 2 + 1) * CH + c8 + k];
         }
-        for (int c8 = 0; c8 < CH; c8 += 8) {
-            u16 zo[8], ao[8];
-            for (int k = 0; k < 8; ++k) {
-                const int co = c8 + k;
-                float s = 0.f;
-                for (int t = 0; t < 9; ++t)
-                    s += in[t][0] * w[(t * 2 + 0) * CH + co] + in[t][1] * w[(t * 2 + 1) * CH + co];
-                s += w[9 * 2 * CH + co];
-                zo[k] = f2bf(s);
-                ao[k] = f2bf(gelu(s));
-            }
-            *reinterpret_cast<uint4 *>(z + p * CH + c8) = *reinterpret_cast<const uint4 *>(zo);
-            *reinterpret_cast<uint4 *>(a + p * CH + c8) = *reinterpret_cast<const uint4 *>(ao);
+        u16 zo[8], ao[8];
+        for (int k = 0; k < 8; ++k) {
+            zo[k] = f2bf(acc[k]);
+            ao[k] = f2bf(gelu(acc[k]));
         }
+        *reinterpret_cast<uint4 *>(z + p * CH + c8) = *reinterpret_cast<const uint4 *>(zo);
+        *reinterpret_cast<uint4 *>(a + p * CH + c8) = *reinterpret_cast<const uint4 *>(ao);
     }
 }
 
 // ------------------------------------------------- layers 1..4 (MFMA)
-// One block: a segment of TPX pixels of one row x 64 output channels; wave
-// w computes pixels 32w..32w+31 (two 16-pixel M tiles) x 4 N tiles of 16
-// channels. K = 9 taps x 64 input channels, in chunks of 16. The input rows
+// One block: a segment of CT = 32 MT pixels of one row x 64 output
+// channels; wave w computes pixels 16 MT w .. (one or two 16-pixel M tiles)
+// x 4 N tiles of 16 channels. K = 9 taps x 64 input channels, in chunks of 16. The input rows
 // y-1..y+1 (with a one-pixel periodic halo) are staged in LDS, the weights
 // tap by tap (double-buffered).
 //   mode 0: out = acc + bias -> zo (BF16), gelu(acc + bias) -> ao (BF16)
 //   mode 1: out = acc * gelu'(zp) -> zo (BF16)   (the backward data pass:
 //           in = dZ_l, wt = the flipped transpose of W_l, zp = z_{l-1})
 // wt: BF16 [9][64 co][64 ci].
+template <int MT>
 __global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
                                                 const u16 *__restrict__ wt,
                                                 const float *__restrict__ bias,
                                                 const u16 *__restrict__ zp, u16 *__restrict__ zo,
                                                 u16 *__restrict__ ao, int H, int W, int mode)
 {
-    __shared__ __align__(16) u16 tin[3 * (TPX + 2) * LP];
+    constexpr int CT = 64 * MT; // pixels of the tile
+    __shared__ __align__(16) u16 tin[3 * (CT + 2) * LP];
     __shared__ __align__(16) u16 tw[2][CH * LP];
-    const int segs = W / TPX;
+    const int segs = W / CT;
     int b = blockIdx.x;
     const int seg = b % segs;
     b /= segs;
     const int y = b % H, n = b / H;
-    const int x0 = seg * TPX;
+    const int x0 = seg * CT;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int q = tid; q < 3 * (TPX + 2) * 8; q += 256) {
+    for (int q = tid; q < 3 * (CT + 2) * 8; q += 256) {
         const int chunk = q & 7, pr = q >> 3;
-        const int r = pr / (TPX + 2), px = pr % (TPX + 2);
+        const int r = pr / (CT + 2), px = pr % (CT + 2);
         const int yy = wrap(y + r - 1, H), xx = wrap(x0 + px - 1, W);
         const uint4 v =
             *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CH + 8 * chunk);
@@ -159,8 +159,8 @@ __global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
     };
     load_w(0, 0);
     __syncthreads();
-    float4_t acc[2][4];
-    for (int m = 0; m < 2; ++m)
+    float4_t acc[MT][4];
+    for (int m = 0; m < MT; ++m)
         for (int j = 0; j < 4; ++j) acc[m][j] = float4_t{0.f, 0.f, 0.f, 0.f};
     const int row16 = lane & 15, kq = 4 * (lane >> 4);
     for (int t = 0; t < 9; ++t) {
@@ -170,17 +170,17 @@ __global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int kk = 16 * c + kq;
-            short4_t av[2], bv[4];
+            short4_t av[MT], bv[4];
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                const int px = 32 * wv + 16 * m + row16 + dx;
-                av[m] = *reinterpret_cast<const short4_t *>(tin + (dy * (TPX + 2) + px) * LP + kk);
+            for (int m = 0; m < MT; ++m) {
+                const int px = 16 * MT * wv + 16 * m + row16 + dx;
+                av[m] = *reinterpret_cast<const short4_t *>(tin + (dy * (CT + 2) + px) * LP + kk);
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 bv[j] = *reinterpret_cast<const short4_t *>(wb + (16 * j + row16) * LP + kk);
 #pragma unroll
-            for (int m = 0; m < 2; ++m)
+            for (int m = 0; m < MT; ++m)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[j], acc[m][j],
@@ -191,14 +191,14 @@ __global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
     // results through LDS (the input tile is free): [px][co] FP32, pitch 68
     constexpr int SOP = CH + 4;
     float *so = reinterpret_cast<float *>(tin);
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < MT; ++m)
         for (int j = 0; j < 4; ++j)
             for (int e = 0; e < 4; ++e) {
-                const int px = 32 * wv + 16 * m + kq + e, co = 16 * j + row16;
+                const int px = 16 * MT * wv + 16 * m + kq + e, co = 16 * j + row16;
                 so[px * SOP + co] = acc[m][j][e] + (bias ? bias[co] : 0.f);
             }
     __syncthreads();
-    for (int q = tid; q < TPX * 8; q += 256) {
+    for (int q = tid; q < CT * 8; q += 256) {
         const int px = q >> 3, c0 = 8 * (q & 7);
         const size_t gi = (((size_t)n * H + y) * W + x0 + px) * CH + c0;
         u16 o1[8], o2[8];
@@ -329,33 +329,42 @@ __global__ __launch_bounds__(256) void k_conv_out(const u16 *__restrict__ a,
     if (threadIdx.x == 0) w[9 * CH] = P[OFF_B5];
     __syncthreads();
     const long long npx = (long long)B * H * W;
-    S2D_GRID_STRIDE(p, npx)
-    {
+    // 8 lanes per pixel, 8 channels each; grid-stride over whole groups so
+    // that every lane of a group takes part in the shuffles
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < npx * 8;
+         q += (long long)gridDim.x * blockDim.x) {
+        const long long p = q >> 3;
+        const int c8 = 8 * (int)(q & 7);
         const int xx = (int)(p % W), yy = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
         float s = 0.f;
         for (int t = 0; t < 9; ++t) {
             const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
-            const u16 *src = a + (((size_t)n * H + y2) * W + x2) * CH;
-            for (int c8 = 0; c8 < CH; c8 += 8) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(src + c8);
-                const u16 *h = reinterpret_cast<const u16 *>(&v);
-                for (int k = 0; k < 8; ++k) s += bf2f(h[k]) * w[t * CH + c8 + k];
-            }
+            const uint4 v = *reinterpret_cast<const uint4 *>(a + (((size_t)n * H + y2) * W + x2) * CH + c8);
+            const u16 *h = reinterpret_cast<const u16 *>(&v);
+            for (int k = 0; k < 8; ++k) s += bf2f(h[k]) * w[t * CH + c8 + k];
         }
-        y[p] = s + w[9 * CH];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        if ((q & 7) == 0) y[p] = s + w[9 * CH];
     }
 }
 
 // Scaled output planes: out[n][yy][xx] (row pitch ld, sample stride sN) =
-// scale * y (the prediction in physical units).
+// scale * y (the prediction in physical units). clip != 0 keeps only the
+// enstrophy sink: tau where omega * tau > 0, zero elsewhere (the closure
+// term -tau then never raises sum omega^2 pointwise).
 __global__ void k_write_out(const float *__restrict__ y, float *__restrict__ out, long long sN,
-                            int ld, float scale, int B, int H, int W)
+                            int ld, float scale, int B, int H, int W,
+                            const float *__restrict__ w, long long wN, int wld, int clip)
 {
     const long long npx = (long long)B * H * W;
     S2D_GRID_STRIDE(p, npx)
     {
         const int xx = (int)(p % W), yy = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
-        out[n * sN + (long long)yy * ld + xx] = scale * y[p];
+        float v = scale * y[p];
+        if (clip && !(v * w[n * wN + (long long)yy * wld + xx] > 0.f)) v = 0.f;
+        out[n * sN + (long long)yy * ld + xx] = v;
     }
 }
 
@@ -516,6 +525,17 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ g, float
     }
 }
 
+// Gather samples idx[b] of a dataset of planes (H x W, dense) into a batch.
+__global__ void k_gather(const float *__restrict__ src, const int *__restrict__ idx,
+                         float *__restrict__ dst, int B, long long plane)
+{
+    S2D_GRID_STRIDE(i, (long long)B * plane)
+    {
+        const int b = (int)(i / plane);
+        dst[i] = src[(long long)idx[b] * plane + (i - (long long)b * plane)];
+    }
+}
+
 thread_local std::string g_cnn_err = "no error";
 
 } // namespace
@@ -524,6 +544,7 @@ struct s2d_cnn {
     int H = 0, W = 0, Bmax = 0, device = 0;
     hipStream_t stream = nullptr;
     float in_scale[2] = {1.f, 1.f}, out_scale = 1.f;
+    int clip = 0; // output mode of s2d_cnn_forward
     long long step = 0;
     // device buffers
     float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
@@ -534,6 +555,10 @@ struct s2d_cnn {
     double *lpart = nullptr;
     size_t part_floats = 0;
     int lblocks = 0;
+    // training set (device): n samples of w, psi, tau planes; batch buffers
+    long long ds_n = 0;
+    float *ds[3] = {}, *bt[3] = {};
+    int *d_idx = nullptr;
     std::string err;
 };
 
@@ -590,16 +615,16 @@ int forward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, i
 {
     const int H = c->H, W = c->W;
     const long long npx = (long long)B * H * W;
-    hipLaunchKernelGGL(k_conv_in, dim3(grid_for(npx, 256)), dim3(256), 0, c->stream, x0, x1, sN,
+    hipLaunchKernelGGL(k_conv_in, dim3(grid_for(npx * 8, 256)), dim3(256), 0, c->stream, x0, x1, sN,
                        ld, c->in_scale[0], c->in_scale[1], c->P, c->z[0], c->a[0], B, H, W);
     CNN_TRY(launched(c, "k_conv_in"));
     for (int l = 1; l <= NMID; ++l) {
-        hipLaunchKernelGGL(k_conv64, dim3(B * H * (W / TPX)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(k_conv64<CMT>, dim3(B * H * (W / (64 * CMT))), dim3(256), 0, c->stream,
                            c->a[l - 1], c->wf + (size_t)(l - 1) * W64, c->P + off_bl(l),
                            (const u16 *)nullptr, c->z[l], c->a[l], H, W, 0);
         CNN_TRY(launched(c, "k_conv64"));
     }
-    hipLaunchKernelGGL(k_conv_out, dim3(grid_for(npx, 256)), dim3(256), 0, c->stream, c->a[NMID],
+    hipLaunchKernelGGL(k_conv_out, dim3(grid_for(npx * 8, 256)), dim3(256), 0, c->stream, c->a[NMID],
                        c->P, c->y, B, H, W);
     return launched(c, "k_conv_out");
 }
@@ -633,7 +658,8 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
         hipLaunchKernelGGL(k_reduce, dim3(grid_for(W64 + CH, 256)), dim3(256), 0, c->stream,
                            c->part, ndw, W64 + CH, c->G + off_wl(l));
         CNN_TRY(launched(c, "k_reduce"));
-        hipLaunchKernelGGL(k_conv64, dim3(B * H * (W / TPX)), dim3(256), 0, c->stream, c->dz[cur],
+        hipLaunchKernelGGL(k_conv64<CMT>, dim3(B * H * (W / (64 * CMT))), dim3(256), 0, c->stream,
+                           c->dz[cur],
                            c->wt + (size_t)(l - 1) * W64, (const float *)nullptr, c->z[l - 1],
                            c->dz[1 - cur], (u16 *)nullptr, H, W, 1);
         CNN_TRY(launched(c, "k_conv64 (backward data)"));
@@ -668,8 +694,9 @@ int s2d_cnn_destroy(s2d_cnn *c)
     if (!c) return S2D_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ps[] = {c->P, c->G, c->M, c->V, c->wf, c->wt, c->dz[0], c->dz[1], c->y, c->dy, c->part,
-                  c->lpart};
+    void *ps[] = {c->P,     c->G,    c->M,     c->V,     c->wf,     c->wt,    c->dz[0],
+                  c->dz[1], c->y,    c->dy,    c->part,  c->lpart,  c->ds[0], c->ds[1],
+                  c->ds[2], c->bt[0], c->bt[1], c->bt[2], c->d_idx};
     for (void *p : ps) (void)hipFree(p);
     for (int l = 0; l <= NMID; ++l) {
         (void)hipFree(c->z[l]);
@@ -760,6 +787,40 @@ int s2d_cnn_set_scales(s2d_cnn *c, float in0, float in1, float out)
     return S2D_OK;
 }
 
+int s2d_cnn_grid(const s2d_cnn *c, int *H, int *W)
+{
+    if (!c || !H || !W) return cfail(nullptr, S2D_ERR_INVALID_ARG, "null argument");
+    *H = c->H;
+    *W = c->W;
+    return S2D_OK;
+}
+
+int s2d_cnn_set_clip(s2d_cnn *c, int clip)
+{
+    if (!c) return cfail(nullptr, S2D_ERR_INVALID_ARG, "null network");
+    c->clip = clip ? 1 : 0;
+    return S2D_OK;
+}
+
+int s2d_cnn_forward_on(s2d_cnn *c, void *stream, int B, const float *w, const float *psi,
+                       long long sN, int ld, float *out, long long oN, int old)
+{
+    CNN_TRY(check_batch(c, B));
+    if (!w || !psi || !out) return cfail(c, S2D_ERR_INVALID_ARG, "null device pointer");
+    CNN_HIP(c, hipSetDevice(c->device));
+    hipStream_t own = c->stream;
+    c->stream = static_cast<hipStream_t>(stream);
+    int r = forward(c, B, w, psi, sN, ld);
+    if (r == S2D_OK) {
+        hipLaunchKernelGGL(k_write_out, dim3(grid_for((long long)B * c->H * c->W, 256)),
+                           dim3(256), 0, c->stream, c->y, out, oN, old, c->out_scale, B, c->H,
+                           c->W, w, sN, ld, c->clip);
+        r = launched(c, "k_write_out");
+    }
+    c->stream = own;
+    return r;
+}
+
 int s2d_cnn_forward(s2d_cnn *c, int B, const float *w, const float *psi, long long sN, int ld,
                     float *out, long long oN, int old)
 {
@@ -768,7 +829,8 @@ int s2d_cnn_forward(s2d_cnn *c, int B, const float *w, const float *psi, long lo
     CNN_HIP(c, hipSetDevice(c->device));
     CNN_TRY(forward(c, B, w, psi, sN, ld));
     hipLaunchKernelGGL(k_write_out, dim3(grid_for((long long)B * c->H * c->W, 256)), dim3(256), 0,
-                       c->stream, c->y, out, oN, old, c->out_scale, B, c->H, c->W);
+                       c->stream, c->y, out, oN, old, c->out_scale, B, c->H, c->W, w, sN, ld,
+                       c->clip);
     CNN_TRY(launched(c, "k_write_out"));
     CNN_HIP(c, hipStreamSynchronize(c->stream));
     return S2D_OK;
@@ -786,6 +848,67 @@ int s2d_cnn_gradient(s2d_cnn *c, int B, const float *w, const float *psi, long l
         CNN_HIP(c, hipMemcpyAsync(grad, c->G, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
         CNN_HIP(c, hipStreamSynchronize(c->stream));
     }
+    return S2D_OK;
+}
+
+int s2d_cnn_set_dataset(s2d_cnn *c, long long n, const float *w, const float *psi,
+                        const float *tau)
+{
+    if (!c || !w || !psi || !tau || n < 1)
+        return cfail(c, S2D_ERR_INVALID_ARG, "null argument or empty dataset");
+    CNN_HIP(c, hipSetDevice(c->device));
+    const size_t plane = (size_t)c->H * c->W;
+    for (int k = 0; k < 3; ++k) {
+        (void)hipFree(c->ds[k]);
+        c->ds[k] = nullptr;
+        if (hipMalloc(&c->ds[k], sizeof(float) * plane * n) != hipSuccess)
+            return cfail(c, S2D_ERR_ALLOC, "dataset of %lld samples does not fit", n);
+        if (!c->bt[k] && hipMalloc(&c->bt[k], sizeof(float) * plane * c->Bmax) != hipSuccess)
+            return cfail(c, S2D_ERR_ALLOC, "batch buffers");
+    }
+    if (!c->d_idx && hipMalloc(&c->d_idx, sizeof(int) * c->Bmax) != hipSuccess)
+        return cfail(c, S2D_ERR_ALLOC, "batch index buffer");
+    const float *src[3] = {w, psi, tau};
+    for (int k = 0; k < 3; ++k)
+        CNN_HIP(c, hipMemcpyAsync(c->ds[k], src[k], sizeof(float) * plane * n,
+                                  hipMemcpyHostToDevice, c->stream));
+    CNN_HIP(c, hipStreamSynchronize(c->stream));
+    c->ds_n = n;
+    return S2D_OK;
+}
+
+int s2d_cnn_train_batch(s2d_cnn *c, int B, const int *idx, float lr, double *loss)
+{
+    CNN_TRY(check_batch(c, B));
+    if (!idx) return cfail(c, S2D_ERR_INVALID_ARG, "null index array");
+    if (c->ds_n < 1) return cfail(c, S2D_ERR_INVALID_ARG, "no dataset (s2d_cnn_set_dataset)");
+    for (int b = 0; b < B; ++b)
+        if (idx[b] < 0 || idx[b] >= c->ds_n)
+            return cfail(c, S2D_ERR_INVALID_ARG, "sample index %d outside [0, %lld)", idx[b], c->ds_n);
+    CNN_HIP(c, hipSetDevice(c->device));
+    CNN_HIP(c, hipMemcpyAsync(c->d_idx, idx, sizeof(int) * B, hipMemcpyHostToDevice, c->stream));
+    const long long plane = (long long)c->H * c->W;
+    for (int k = 0; k < 3; ++k) {
+        hipLaunchKernelGGL(k_gather, dim3(grid_for(B * plane, 256)), dim3(256), 0, c->stream,
+                           c->ds[k], c->d_idx, c->bt[k], B, plane);
+        CNN_TRY(launched(c, "k_gather"));
+    }
+    return s2d_cnn_train_step(c, B, c->bt[0], c->bt[1], plane, c->W, c->bt[2], plane, c->W, lr,
+                              loss);
+}
+
+int s2d_cnn_predict(s2d_cnn *c, long long i, float *host_tau)
+{
+    if (!c || !host_tau) return cfail(c, S2D_ERR_INVALID_ARG, "null argument");
+    if (i < 0 || i >= c->ds_n) return cfail(c, S2D_ERR_INVALID_ARG, "sample %lld outside the dataset", i);
+    const long long plane = (long long)c->H * c->W;
+    CNN_HIP(c, hipSetDevice(c->device));
+    // the scaled prediction into the dy scratch plane
+    CNN_TRY(s2d_cnn_forward(c, 1, c->ds[0] + i * plane, c->ds[1] + i * plane, plane, c->W,
+                            c->dy, plane, c->W));
+    CNN_HIP(c, hipMemcpyAsync(host_tau, c->dy, sizeof(float) * plane, hipMemcpyDeviceToHost,
+                              c->stream));
+    CNN_HIP(c, hipStreamSynchronize(c->stream));
     return S2D_OK;
 }
 

@@ -920,6 +920,74 @@ int set_device(s2d_ctx *c)
     return S2D_OK;
 }
 
+// Closure coupling (s2d_closure_forcing / s2d_closure_steps) ------------
+int check_closure(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net)
+{
+    S2D_TRY(check_ctx(c));
+    S2D_TRY(check_ctx(g));
+    if (!net) return fail(c, S2D_ERR_INVALID_ARG, "null network");
+    int nh = 0, nw = 0;
+    if (s2d_cnn_grid(net, &nh, &nw) != S2D_OK) return fail(c, S2D_ERR_INVALID_ARG, "bad network");
+    if (!c->fused || !c->wave_kernels || c->nruns != 1 || c->precision != S2D_FP32 ||
+        g->precision != S2D_FP32 || g->g.dealias != S2D_NO_DEALIASING || g->g.nx != c->g.nx ||
+        g->g.ny != c->g.ny || nh != g->g.nyG || nw != g->g.nxG || g->device != c->device)
+        return fail(c, S2D_ERR_UNSUPPORTED,
+                    "closure coupling: an FP32 fused run, an FP32 unpadded grid of its size and a "
+                    "network of that grid");
+    return S2D_OK;
+}
+
+// The closure forcing sequence on c's stream, no waits: (optionally the
+// compact state into W / PSI,) the cutoff sampling onto g, two c2r, the
+// network, r2c, the restriction back, the pack into the forcing buffer.
+int enqueue_closure_forcing(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, bool unpack)
+{
+    if (unpack) S2D_TRY(op_unpack<float>(c, 0));
+    hipStream_t gs = g->stream;
+    g->stream = c->stream; // the whole sequence on one stream
+    const long long kc = c->g.nx / 2;
+    auto restrict_ = [&](s2d_ctx *src, int sf, s2d_ctx *dst, int df) {
+        const size_t n = nc_of(dst);
+        hipLaunchKernelGGL((k_restrict<float, float>), dim3(grid_blocks(n, 256)), dim3(256), 0,
+                           c->stream, F<float>(src, sf), src->g.cISize, src->g.nyG,
+                           F<float>(dst, df), dst->g.cISize, dst->g.nyG, kc * kc);
+        return hipGetLastError() == hipSuccess ? S2D_OK
+                                               : fail(c, S2D_ERR_HIP, "launch of k_restrict failed");
+    };
+    int r = restrict_(c, S2D_FIELD_W, g, S2D_FIELD_W);
+    if (!r) r = restrict_(c, S2D_FIELD_PSI, g, S2D_FIELD_PSI);
+    if (!r) r = op_c2r<float>(g, S2D_FIELD_W);
+    if (!r) r = op_c2r<float>(g, S2D_FIELD_PSI);
+    if (!r) {
+        const int ld = g->g.rISize; // the real layout of the grid context
+        if (s2d_cnn_forward_on(net, c->stream, 1, static_cast<const float *>(g->field[S2D_FIELD_W]),
+                               static_cast<const float *>(g->field[S2D_FIELD_PSI]), 0, ld,
+                               static_cast<float *>(g->field[S2D_FIELD_TMP]), 0, ld) != S2D_OK)
+            r = fail(c, S2D_ERR_HIP, "closure network: %s", s2d_cnn_last_error(net));
+    }
+    if (!r) r = op_r2c<float>(g, S2D_FIELD_TMP);
+    if (!r) r = restrict_(g, S2D_FIELD_TMP, c, S2D_FIELD_TMP);
+    g->stream = gs;
+    S2D_TRY(r);
+    // the kept modes of TMP into the forcing buffer (as s2d_set_forcing)
+    void *keep = c->Wn, *wslot = c->field[S2D_FIELD_W];
+    c->Wn = c->forcing;
+    c->field[S2D_FIELD_W] = c->field[S2D_FIELD_TMP];
+    r = op_pack<float>(c, 0);
+    c->field[S2D_FIELD_W] = wslot;
+    c->Wn = keep;
+    return r;
+}
+
+void drop_closure_graphs(s2d_ctx *c)
+{
+    for (auto &gx : c->cl_graph) {
+        if (gx) (void)hipGraphExecDestroy(gx);
+        gx = nullptr;
+    }
+}
+
+
 } // namespace
 
 namespace {
@@ -1219,6 +1287,7 @@ int s2d_destroy(s2d_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    drop_closure_graphs(c);
     for (int f = 0; f < S2D_NFIELDS; ++f) (void)hipFree(c->field[f]);
     void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->d_nu, c->forcing, c->col.tw,
                   c->rowh.tw, c->rowf.tw, c->Wn, c->Ws, c->G[0], c->G[1], c->G[2], c->H[0],
@@ -1606,10 +1675,88 @@ int s2d_set_forcing(s2d_ctx *c, int f)
     return sync(c);
 }
 
+int s2d_closure_forcing(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net)
+{
+    S2D_TRY(check_closure(c, g, net));
+    S2D_TRY(set_device(c));
+    S2D_TRY(ensure_ref(c));
+    S2D_TRY(ensure_ref(g));
+    S2D_TRY(sync(g)); // nothing of g's own stream may still run
+    if (!c->forcing) S2D_HIP(c, hipMalloc(&c->forcing, (size_t)c->sW * c->celt));
+    S2D_TRY(enqueue_closure_forcing(c, g, net, false));
+    c->forcing_on = true;
+    return sync(c);
+}
+
+int s2d_closure_steps(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, int nsteps, double nu, double cfl,
+                      int every)
+{
+    S2D_TRY(check_closure(c, g, net));
+    if (nsteps < 0 || every < 1) return fail(c, S2D_ERR_INVALID_ARG, "bad step count or period");
+    if (!(cfl > 0.0)) return fail(c, S2D_ERR_INVALID_ARG, "closure steps take a CFL time step");
+    S2D_TRY(set_device(c));
+    S2D_TRY(ensure_compact(c));
+    S2D_TRY(sync(g));
+    if (!c->forcing) S2D_HIP(c, hipMalloc(&c->forcing, (size_t)c->sW * c->celt));
+    c->forcing_on = true;
+    const bool fresh = c->cl_net == net && c->cl_grid == g && c->cl_nu == nu && c->cl_cfl == cfl;
+    if (!fresh) drop_closure_graphs(c);
+    for (int k = 0; k < 2 && !c->cl_graph[k]; ++k) {
+        // k = 0: forcing refresh + step; k = 1: step with the frozen forcing
+        hipGraph_t gr = nullptr;
+        S2D_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = k == 0 ? enqueue_closure_forcing(c, g, net, true) : S2D_OK;
+        if (!r) r = fused_steps<float>(c, 1, nu, 0.0, cfl);
+        const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
+        if (r || ec != hipSuccess || !gr) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return r ? r : fail(c, S2D_ERR_HIP, "capture of the closure step failed");
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->cl_graph[k], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) {
+            c->cl_graph[k] = nullptr;
+            return fail(c, S2D_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        }
+    }
+    c->cl_net = net;
+    c->cl_grid = g;
+    c->cl_nu = nu;
+    c->cl_cfl = cfl;
+    for (int st = 0; st < nsteps; ++st)
+        S2D_HIP(c, hipGraphLaunch(c->cl_graph[st % every == 0 ? 0 : 1], c->stream));
+    c->ref_stale = true;
+    const size_t n = (size_t)c->K * c->KP;
+    hipLaunchKernelGGL(k_check_finite<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
+                       (const cpx<float> *)c->Wn, n, (long long)n, c->d_sc);
+    S2D_LAUNCHED(c, "k_check_finite");
+    DevScalars sc;
+    S2D_HIP(c, hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    S2D_TRY(sync(c));
+    double vmax;
+    std::memcpy(&vmax, &sc.vmax_bits, sizeof(double));
+    if (sc.bad || !(vmax == vmax) || vmax > 1.0e300)
+        return fail(c, S2D_ERR_NONFINITE, "non-finite values in the state");
+    return S2D_OK;
+}
+
 int s2d_clear_forcing(s2d_ctx *c)
 {
     S2D_TRY(check_ctx(c));
     c->forcing_on = false;
+    return S2D_OK;
+}
+
+int s2d_get_time(s2d_ctx *c, double *time, long long *nadv, double *dt)
+{
+    S2D_TRY(check_ctx(c));
+    S2D_TRY(set_device(c));
+    DevScalars sc;
+    S2D_HIP(c, hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    S2D_TRY(sync(c));
+    if (time) *time = sc.time;
+    if (nadv) *nadv = sc.nadv;
+    if (dt) *dt = sc.dt;
     return S2D_OK;
 }
 

@@ -71,6 +71,11 @@ struct s2d_ctx {
     bool member = false;              // a size class of an ensemble
     void *forcing = nullptr;          // compact-layout RHS forcing (s2d_set_forcing)
     bool forcing_on = false;
+    // captured closure steps (s2d_closure_steps): [0] forcing refresh + step,
+    // [1] step; valid for the recorded grid, network, nu and cfl
+    hipGraphExec_t cl_graph[2] = {nullptr, nullptr};
+    void *cl_grid = nullptr, *cl_net = nullptr;
+    double cl_nu = 0.0, cl_cfl = 0.0;
 
     // fused compact state
     bool fused = false;

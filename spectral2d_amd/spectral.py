@@ -251,6 +251,12 @@ class Spectral2D:
         self._c(L.lib().s2d_profile_end(self._h, ms, cnt))
         return [(ms[k], cnt[k]) for k in range(3)]
 
+    def get_time(self):
+        """(time, steps taken, dt of the last step)."""
+        t, n, d = C.c_double(), C.c_longlong(), C.c_double()
+        self._c(L.lib().s2d_get_time(self._h, C.byref(t), C.byref(n), C.byref(d)))
+        return t.value, n.value, d.value
+
     def set_time(self, time: float = 0.0, nadv: int = 0):
         self._c(L.lib().s2d_set_time(self._h, float(time), int(nadv)))
 

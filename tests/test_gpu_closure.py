@@ -1,0 +1,87 @@
+"""The closure coupling on the GPU: the one-call s2d_closure_forcing equals
+the per-routine sequence (restrict, transforms, network, transform,
+restrict, s2d_set_forcing) bitwise, and a forcing of zero network output
+leaves the run unchanged."""
+import numpy as np
+import pytest
+
+from oracle import cnn as ocnn
+from spectral2d_amd.closure import ClosedRun
+from spectral2d_amd.cnn import ClosureNet
+from spectral2d_amd.spectral import FP32, Spectral2D
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(N, zero=False):
+    net = ClosureNet(N, N, 1)
+    p = ocnn.init_params(2).astype(np.float32)
+    if zero:
+        p[:] = 0.0
+    net.set_params(p)
+    net.set_scales(0.2, 5.0, 0.05)
+    return net
+
+
+def test_fused_forcing_equals_per_routine_sequence():
+    N = 128
+    print(f"[closure] fused vs per-routine forcing, N={N}", flush=True)
+    net = _net(N)
+    a, b = ClosedRun(N, net), ClosedRun(N, net)
+    for r in (a, b):
+        r.sim.SetTurbulence(11, 8.0, 0.5)
+    for _ in range(3):
+        a.closure_forcing()
+        b.closure_forcing_steps()
+        a.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+        b.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+    assert np.array_equal(a.sim.get_field("w"), b.sim.get_field("w"))
+    for r in (a, b):
+        r.close()
+    net.close()
+
+
+def test_zero_closure_is_the_plain_run():
+    N = 128
+    print("[closure] zero network output", flush=True)
+    net = _net(N, zero=True)
+    a = ClosedRun(N, net)
+    p = Spectral2D(N, precision=FP32)
+    for s in (a.sim, p):
+        s.SetTurbulence(12, 8.0, 0.5)
+    for _ in range(3):
+        a.closure_forcing()
+        a.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+        p.rk3_steps(1, 1e-4, 0.0, 0.5)
+    # the forcing is exactly zero: x - 0 == x in every stage
+    assert np.array_equal(a.sim.get_field("w"), p.get_field("w"))
+    a.close()
+    p.close()
+    net.close()
+
+
+def test_captured_closure_steps_equal_eager_steps():
+    """s2d_closure_steps (hipGraph replay, forcing refreshed every 2 steps)
+    equals the same sequence of eager calls bitwise."""
+    N = 128
+    print("[closure] captured closure steps vs eager", flush=True)
+    net = _net(N)
+    a, b = ClosedRun(N, net, every=2), ClosedRun(N, net, every=2)
+    for r in (a, b):
+        r.sim.SetTurbulence(13, 8.0, 0.5)
+    a.steps(5, 1e-4, 0.5)
+    for k in range(5):
+        if k % 2 == 0:
+            b.closure_forcing()
+        b.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+    assert np.array_equal(a.sim.get_field("w"), b.sim.get_field("w"))
+    assert a.sim.get_time() == b.sim.get_time()
+    a.steps(3, 1e-4, 0.5)  # replays the cached graphs
+    for k in range(3):
+        if k % 2 == 0:
+            b.closure_forcing()
+        b.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+    assert np.array_equal(a.sim.get_field("w"), b.sim.get_field("w"))
+    for r in (a, b):
+        r.close()
+    net.close()
