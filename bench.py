@@ -13,6 +13,13 @@ library's stream around every launch in the timed region. `cpu_baseline`
 times the CPU oracle (a C restatement of the reference algorithm) on the
 host cores for a bounded number of steps of the same workload.
 
+It also carries `closure`: the learned-closure speedup at the 5 % spectrum
+bound. Held-out members start 256^2 runs (the shipped closure network,
+refreshed every 3 steps) from the filtered 2048^2 DNS at t = 0.5; at t = 1
+the ensemble-mean enstrophy spectrum is compared with the filtered DNS, and
+the GPU time of the 2048^2 solve over [0.5, 1] is divided by the closure
+run's (both wall-clock on this GPU, one stream).
+
 The same line carries `ensemble`: the batched FP32 ensemble of 256
 independent decaying-turbulence runs (N drawn from {128, 256, 512}, k0 in
 [4, 16], nu log-uniform in [1e-4, 1e-3], seeds 1000 + m, per-run CFL 0.5),
@@ -197,6 +204,30 @@ def run_ensemble(steps: int, warmup: int, cfl: float):
     }
 
 
+def run_closure(members: int):
+    from spectral2d_amd.closure_run import eval_args, evaluate, load_net
+    a = eval_args(eval_runs=members)
+    net = load_net()
+    e = evaluate(a, net, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    net.close()
+    if "err_closure" not in e:
+        return {"metric": "closure speedup @5% spectrum err", "value": None, "unstable": e}
+    return {
+        "metric": "closure speedup @5% spectrum err",
+        "value": round(e["speedup_vs_dns"], 2),
+        "unit": "x (GPU time, 2048^2 DNS / 256^2 closure run, t 0.5 -> 1.0)",
+        "spectrum_err_closure": round(e["err_closure"], 5),
+        "spectrum_err_plain": round(e["err_plain"], 5),
+        "within_5pct": bool(e["err_closure"] <= 0.05),
+        "members": members,
+        "gpu_s_dns": round(e["gpu_s_dns"], 3), "gpu_s_closure": round(e["gpu_s_closure"], 4),
+        "steps": e["steps"],
+        "config": {"workload": "held-out seeds 3000.., 2048^2 DNS (FP32, nu 2e-5, k0 12, CFL 0.5) "
+                               "vs 256^2 with the shipped CNN closure, refreshed every 3 steps, "
+                               "shell guard on; Z(k) = k^2 E(k) over shells 1..128"},
+    }
+
+
 def cpu_baseline(N: int, nu: float, cfl: float, budget_s: float):
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -246,6 +277,8 @@ def main(argv=None):
     ap.add_argument("--workload", choices=("all", "c1", "ensemble"), default="all",
                     help="c1: the 1024^2 run only; ensemble: the ensemble only (its JSON "
                          "object is printed alone); all: both on one line")
+    ap.add_argument("--closure-members", type=int, default=4,
+                    help="held-out members of the closure evaluation (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--ens-warmup", type=int, default=10)
     args = ap.parse_args(argv)
@@ -339,6 +372,8 @@ def main(argv=None):
     }
     if args.workload == "all":
         out["ensemble"] = run_ensemble(args.ens_steps, args.ens_warmup, args.cfl)
+        if args.closure_members > 0:
+            out["closure"] = run_closure(args.closure_members)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(N, args.nu, args.cfl, args.cpu_budget)
     print(json.dumps(out), flush=True)

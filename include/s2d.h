@@ -219,6 +219,10 @@ int s2d_closure_forcing(s2d_ctx *ctx, s2d_ctx *grid, struct s2d_cnn *net);
  * network, nu or cfl change). Checked for NaN/Inf at the end. */
 int s2d_closure_steps(s2d_ctx *ctx, s2d_ctx *grid, struct s2d_cnn *net, int nsteps, double nu,
                       double cfl, int every);
+/* on != 0: the closure forcing is zeroed on every shell floor(|k| + 1/2) =
+ * b where -F would raise the shell's enstrophy (sum over the shell of
+ * Re(conj(w_hat) F) < 0): the closure keeps its net drain, shell by shell. */
+int s2d_set_closure_guard(s2d_ctx *ctx, int on);
 
 /* Checkpoint / restart (the neighbour of WriteRestart / ReadRestart,
  * src/IO.F90:98-143, 306-424, which write HDF5): a little-endian binary

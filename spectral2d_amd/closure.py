@@ -76,13 +76,16 @@ class ClosedRun:
     """A coarse (Nc, 3/2 rule) run with the CNN closure term."""
 
     def __init__(self, Nc: int, net: ClosureNet | None, precision: int = FP32,
-                 every: int = 1):
+                 every: int = 1, guard: bool = False):
         self.Nc = Nc
         self.kc = Nc // 2
         self.net = net
         self.every = every  # closure evaluations: every `every` steps (frozen between)
         self.sim = Spectral2D(Nc, precision=precision)
         self.grid = Spectral2D(Nc, dealias=NO_DEALIASING, precision=precision)
+        if guard:
+            from . import _lib as L
+            L.check(L.lib().s2d_set_closure_guard(self.sim._h, 1), self.sim._h)
         if net is not None and precision != FP32:
             raise ValueError("the closure network reads FP32 planes")
 

@@ -96,7 +96,29 @@ def init_params(seed, n):
     return p
 
 
-def evaluate(a, net):
+SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                       "closure_cnn_256.npz")
+
+
+def load_net(path: str = SHIPPED, nc: int = 256):
+    """A batch-1 closure network with stored parameters and scales."""
+    from .cnn import ClosureNet
+    d = np.load(path)
+    net = ClosureNet(nc, nc, 1)
+    net.set_params(d["params"])
+    net.set_scales(*[float(v) for v in d["scales"]])
+    return net
+
+
+def eval_args(**kw):
+    """The evaluation settings of main() (C4 defaults), overridable."""
+    base = dict(nf=2048, nc=256, nu=2e-5, k0=12.0, cfl=0.5, eval_runs=64, eval_seed0=3000,
+                t0=0.5, t1=1.0, closure_every=3, guard=1)
+    base.update(kw)
+    return argparse.Namespace(**base)
+
+
+def evaluate(a, net, log=log):
     from .closure import ClosedRun, FilteredSampler, Timer, advance_to, enstrophy_spectrum
     from .closure import spectrum_error
     kmax = a.nc // 2
@@ -105,11 +127,12 @@ def evaluate(a, net):
     steps = {"dns": 0, "plain": 0, "closure": 0}
     unstable = {}
     for m in range(a.eval_runs):
-        s = make_dns(a.nf, 3000 + m, a.k0, a.nu)
+        s = make_dns(a.nf, a.eval_seed0 + m, a.k0, a.nu)
         advance_to(s, a.t0, a.nu, a.cfl)
         runs = {}
         for kind in ("plain", "closure"):
-            r = ClosedRun(a.nc, net if kind == "closure" else None, every=a.closure_every)
+            r = ClosedRun(a.nc, net if kind == "closure" else None, every=a.closure_every,
+                          guard=bool(a.guard))
             s.restrict_to("w", r.sim, "w", kmax)
             r.sim.ComputePsi("w", "psi")
             r.sim.set_time(s.get_time()[0], 0)
@@ -161,31 +184,32 @@ def main(argv=None):
     ap.add_argument("--nu", type=float, default=2e-5)
     ap.add_argument("--k0", type=float, default=12.0)
     ap.add_argument("--cfl", type=float, default=0.5)
-    ap.add_argument("--train-runs", type=int, default=16)
+    ap.add_argument("--train-runs", type=int, default=32)
     ap.add_argument("--t-spin", type=float, default=0.4)
     ap.add_argument("--t-end", type=float, default=1.0)
-    ap.add_argument("--every", type=int, default=20)
-    ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--every", type=int, default=10)
+    ap.add_argument("--iters", type=int, default=2500)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--eval-runs", type=int, default=8)
+    ap.add_argument("--eval-runs", type=int, default=64)
+    ap.add_argument("--eval-seed0", type=int, default=3000)
     ap.add_argument("--t0", type=float, default=0.5)
     ap.add_argument("--t1", type=float, default=1.0)
-    ap.add_argument("--clip", type=int, default=1,
+    ap.add_argument("--clip", type=int, default=0,
                     help="1: keep only the enstrophy-sink part of tau (omega tau > 0)")
-    ap.add_argument("--closure-every", type=int, default=1,
+    ap.add_argument("--guard", type=int, default=1,
+                    help="1: zero the closure forcing on backscattering shells")
+    ap.add_argument("--closure-every", type=int, default=3,
                     help="closure evaluations every this many coarse steps")
-    ap.add_argument("--weights", help="skip data and training: evaluate these weights")
+    ap.add_argument("--weights", help="skip data and training: evaluate these weights (e.g. "
+                    "spectral2d_amd/data/closure_cnn_256.npz)")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     from .cnn import ClosureNet
     metrics = {"config": vars(a)}
     if a.weights:
-        d = np.load(a.weights)
-        net = ClosureNet(a.nc, a.nc, 1)
-        net.set_params(d["params"])
-        net.set_scales(*[float(v) for v in d["scales"]])
+        net = load_net(a.weights, a.nc)
     else:
         log("generating filtered DNS data")
         t0 = time.perf_counter()

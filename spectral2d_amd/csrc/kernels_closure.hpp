@@ -82,4 +82,53 @@ __global__ void k_axpby(const cpx<T> *__restrict__ x, cpx<T> *__restrict__ y, do
     }
 }
 
+// Shell guard of a closure forcing F (compact layout, column kx at
+// kx * KP, row c: ky = c for c <= N/2, ky = c - N - 1 above): the term -F
+// changes the enstrophy of shell b (kx^2 + ky^2 in [b^2 - b + 1, b^2 + b])
+// at the rate -2 sum w_kx Re(conj(W) F) (w = 1 at kx = 0, 2 elsewhere);
+// shells where that rate is positive (backscatter) get F = 0. One block per
+// shell, fixed reduction order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_shell_guard(cpx<T> *__restrict__ Fc,
+                                                     const cpx<T> *__restrict__ Wc, int N,
+                                                     int K, int KP)
+{
+    __shared__ double red[256];
+    const long long b = blockIdx.x;
+    const long long lo = b == 0 ? 0 : b * b - b + 1, hi = b == 0 ? 0 : b * b + b;
+    const int h = N / 2;
+    auto isqrt = [](long long m) { // largest r with r * r <= m
+        long long r = (long long)sqrt((double)m);
+        while (r * r > m) --r;
+        while ((r + 1) * (r + 1) <= m) ++r;
+        return r;
+    };
+    auto visit = [&](auto &&f) {
+        for (int kx = threadIdx.x; kx < K && (long long)kx * kx <= hi; kx += blockDim.x) {
+            const long long kx2 = (long long)kx * kx;
+            const long long a_hi = isqrt(hi - kx2);
+            const long long need = lo - kx2;
+            const long long a_lo = need <= 0 ? 0 : isqrt(need - 1) + 1;
+            for (long long a = a_lo; a <= a_hi && a <= h; ++a)
+                for (int sg = 0; sg < (a > 0 ? 2 : 1); ++sg) {
+                    const int ky = (int)(sg == 0 ? a : -a);
+                    const int c = ky >= 0 ? ky : ky + N + 1;
+                    f((size_t)kx * KP + c, kx);
+                }
+        }
+    };
+    double acc = 0.0;
+    visit([&](size_t i, int kx) {
+        const cpx<T> w = Wc[i], f = Fc[i];
+        acc += (kx == 0 ? 1.0 : 2.0) * ((double)w.re * (double)f.re + (double)w.im * (double)f.im);
+    });
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (red[0] < 0.0) visit([&](size_t i, int) { Fc[i] = mk<T>((T)0, (T)0); });
+}
+
 } // namespace s2d

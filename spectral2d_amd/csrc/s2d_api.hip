@@ -976,6 +976,13 @@ int enqueue_closure_forcing(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, bool unpack)
     r = op_pack<float>(c, 0);
     c->field[S2D_FIELD_W] = wslot;
     c->Wn = keep;
+    if (!r && c->closure_guard) {
+        const int nshell = (int)std::floor(std::sqrt(2.0) * (c->N / 2)) + 2;
+        hipLaunchKernelGGL(k_shell_guard<float>, dim3(nshell), dim3(256), 0, c->stream,
+                           (cpx<float> *)c->forcing, (const cpx<float> *)c->Wn, c->N, c->K,
+                           c->KP);
+        if (hipGetLastError() != hipSuccess) r = fail(c, S2D_ERR_HIP, "launch of k_shell_guard failed");
+    }
     return r;
 }
 
@@ -1737,6 +1744,15 @@ int s2d_closure_steps(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, int nsteps, double n
     std::memcpy(&vmax, &sc.vmax_bits, sizeof(double));
     if (sc.bad || !(vmax == vmax) || vmax > 1.0e300)
         return fail(c, S2D_ERR_NONFINITE, "non-finite values in the state");
+    return S2D_OK;
+}
+
+int s2d_set_closure_guard(s2d_ctx *c, int on)
+{
+    S2D_TRY(check_ctx(c));
+    const bool g = on != 0;
+    if (g != c->closure_guard) drop_closure_graphs(c);
+    c->closure_guard = g;
     return S2D_OK;
 }
 

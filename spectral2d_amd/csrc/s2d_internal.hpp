@@ -76,6 +76,7 @@ struct s2d_ctx {
     hipGraphExec_t cl_graph[2] = {nullptr, nullptr};
     void *cl_grid = nullptr, *cl_net = nullptr;
     double cl_nu = 0.0, cl_cfl = 0.0;
+    bool closure_guard = false; // zero the closure forcing on backscattering shells
 
     // fused compact state
     bool fused = false;
