@@ -62,15 +62,23 @@ KERNEL_TAGS = {"c1": ("k2_passA<", "k2_passB<", "k2_passC<"),
                "ensemble": ("k3_ens_passA<", "k3_ens_passB<", "k3_ens_passC<")}
 
 
+# the sources the three pass kernels and their launch shapes are compiled from
+PASS_SOURCES = ("fft_block.hpp", "fft_wave.hpp", "kernels_ref.hpp", "kernels_fused.hpp",
+                "kernels_fused_v2.hpp", "kernels_ensemble.hpp", "s2d_internal.hpp",
+                "s2d_ensemble.inc")
+
+
 def kernel_source_hash() -> str:
-    """Hash of the HIP sources: measured traffic is only reported for the
-    kernels it was measured on."""
+    """Hash of the pass kernels' sources (code lines, without // comments and
+    blank lines): measured traffic is only reported for the kernels it was
+    measured on."""
     import hashlib
     h = hashlib.sha256()
     d = os.path.join(ROOT, "spectral2d_amd", "csrc")
-    for fn in sorted(os.listdir(d)):
-        with open(os.path.join(d, fn), "rb") as f:
-            h.update(fn.encode() + b"\0" + f.read())
+    for fn in PASS_SOURCES:
+        with open(os.path.join(d, fn)) as f:
+            code = [ln.split("//", 1)[0].rstrip() for ln in f]
+        h.update(fn.encode() + b"\0" + "\n".join(c for c in code if c.strip()).encode())
     return h.hexdigest()[:16]
 
 

@@ -9,9 +9,9 @@
 // radix-E butterfly per lane, in registers, whose outputs X[t + TG*m] are
 // again the lane's own elements.
 //
-// The LDS slice is padded with one slot after every 128 bytes, which keeps
-// the strided Stockham writes and the contiguous reads free of bank
-// conflicts. Groups inside one wave synchronise with wave-scope fences;
+// LDS slots are permuted inside aligned blocks of 16 elements (LdsPad), which
+// spreads the strided Stockham writes and the contiguous reads over the
+// banks without padding. Groups inside one wave synchronise with wave-scope fences;
 // multi-wave groups use workgroup barriers, which every thread of the
 // workgroup must reach the same number of times.
 //
