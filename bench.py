@@ -212,6 +212,50 @@ def run_ensemble(steps: int, warmup: int, cfl: float):
     }
 
 
+def run_c1_fp32(N: int, steps: int, warmup: int, nu: float, cfl: float):
+    """The same decaying-turbulence workload with an FP32 context."""
+    from spectral2d_amd.spectral import FP32, Spectral2D
+    sim = Spectral2D(N, precision=FP32)
+    sim.SetTurbulence(seed=1, k0=10.0, ke=0.5)
+    sim.set_time(0.0, 0)
+    sim.rk3_steps(warmup, nu, 0.0, cfl)
+    t0 = time.perf_counter()
+    sim.rk3_steps(steps, nu, 0.0, cfl)
+    el = time.perf_counter() - t0
+    sim.close()
+    bps = 3 * sum(alg_bytes_per_launch(N, 4))
+    return {"metric": "Mpts·RK3-steps/s (FP32 context)", "value": round(N * N / 1e6 * steps / el, 3),
+            "unit": UNIT, "steps": steps, "ms_per_step": round(el / steps * 1e3, 5), "dtype": "f32",
+            "alg_gbs": round(bps * steps / el / 1e9, 1),
+            "config": {"workload": f"decaying 2D turbulence {N}^2 (padded {3 * N // 2}^2), FP32, "
+                                   f"CFL {cfl}, nu {nu}"}}
+
+
+def run_host_exchange(N: int, steps: int, nu: float, cfl: float):
+    """Steps as the Fortran IntegrateOneStep makes them by default: one step,
+    then wC and psiC copied to host buffers in the reference layout (the
+    only fields valid after a step)."""
+    import numpy as np
+    from spectral2d_amd.spectral import FP64, Spectral2D
+    sim = Spectral2D(N, precision=FP64)
+    sim.SetTurbulence(seed=1, k0=10.0, ke=0.5)
+    sim.set_time(0.0, 0)
+    w = np.empty(2 * sim.cISize * sim.nyG)
+    p = np.empty_like(w)
+    sim.rk3_steps(2, nu, 0.0, cfl)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.rk3_steps(1, nu, 0.0, cfl)
+        sim.get_field("w", out=w)
+        sim.get_field("psi", out=p)
+    el = time.perf_counter() - t0
+    sim.close()
+    return {"metric": "Mpts·RK3-steps/s with wC, psiC copied to the host after every step",
+            "value": round(N * N / 1e6 * steps / el, 3), "unit": UNIT, "steps": steps,
+            "ms_per_step": round(el / steps * 1e3, 5), "dtype": "f64",
+            "host_bytes_per_step": 2 * w.nbytes}
+
+
 def run_closure(members: int):
     from spectral2d_amd.closure_run import eval_args, evaluate, load_net
     a = eval_args(eval_runs=members)
@@ -379,6 +423,8 @@ def main(argv=None):
         "kernels": kernel_table,
     }
     if args.workload == "all":
+        out["fp32"] = run_c1_fp32(N, args.steps, args.warmup, args.nu, args.cfl)
+        out["host_exchange"] = run_host_exchange(N, 100, args.nu, args.cfl)
         out["ensemble"] = run_ensemble(args.ens_steps, args.ens_warmup, args.cfl)
         if args.closure_members > 0:
             out["closure"] = run_closure(args.closure_members)

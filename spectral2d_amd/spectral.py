@@ -113,8 +113,14 @@ class Spectral2D:
             raise ValueError(f"field buffer must hold {self.ndoubles} doubles, got {buf.size}")
         self._c(L.lib().s2d_set_field_host(self._h, _fid(f), _dp(buf)))
 
-    def get_field(self, f) -> np.ndarray:
-        out = self.field()
+    def get_field(self, f, out: np.ndarray | None = None) -> np.ndarray:
+        """Slot f in the reference layout (into `out` when given: a
+        contiguous float64 array of 2*cISize*nyG values)."""
+        if out is None:
+            out = self.field()
+        elif out.dtype != np.float64 or out.size != 2 * self.cISize * self.nyG \
+                or not out.flags.c_contiguous:
+            raise ValueError("out must be contiguous float64 of 2*cISize*nyG values")
         self._c(L.lib().s2d_get_field_host(self._h, _fid(f), _dp(out)))
         return out
 
