@@ -257,6 +257,10 @@ int s2d_set_taylor_green(s2d_ctx *ctx);
 #define S2D_KERNEL_PASS_C 2     /* column forward + RHS + RK3 stage */
 #define S2D_NKERNELS 3
 int s2d_profile_begin(s2d_ctx *ctx, int max_launches);
+/* Within a timing session: bracket only the launches of every every-th step
+ * (counted from s2d_profile_begin), so that the event pairs' dispatch gaps
+ * stay out of the other steps. */
+int s2d_profile_every(s2d_ctx *ctx, int every);
 int s2d_profile_end(s2d_ctx *ctx, double *ms, long long *count);
 
 /* ---- ensemble of independent runs ----

@@ -43,7 +43,7 @@ EXPORTS = (
     "s2d_compute_psi", "s2d_compute_velocity", "s2d_compute_vorticity",
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
     "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_get_time", "s2d_profile_begin",
-    "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
+    "s2d_profile_every", "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
     "s2d_spectrum_bins", "s2d_spectrum", "s2d_ensemble_spectrum",
     "s2d_checkpoint_write", "s2d_checkpoint_read", "s2d_restrict", "s2d_axpby",
     "s2d_set_forcing", "s2d_clear_forcing",
@@ -138,6 +138,7 @@ def lib():
     L.s2d_set_time.argtypes = [vp, C.c_double, C.c_longlong]
     L.s2d_get_time.argtypes = [vp, dp, P(C.c_longlong), dp]
     L.s2d_profile_begin.argtypes = [vp, C.c_int]
+    L.s2d_profile_every.argtypes = [vp, C.c_int]
     L.s2d_profile_end.argtypes = [vp, dp, P(C.c_longlong)]
     L.s2d_set_turbulence.argtypes = [vp, C.c_ulonglong, C.c_double, C.c_double]
     L.s2d_set_taylor_green.argtypes = [vp]

@@ -15,10 +15,12 @@
 // multi-wave groups use workgroup barriers, which every thread of the
 // workgroup must reach the same number of times.
 //
-// Twiddles come from the FP64-generated table tw[e] = exp(-2 pi i e / n)
-// of the transform length (rounded to T); the in-register DFTs use the same
-// table at stride n/E, read at lane-uniform addresses.
+// Per-lane twiddles come from the FP64-generated table tw[e] =
+// exp(-2 pi i e / n) of the transform length (rounded to T); the twiddles
+// inside the in-register DFTs are compile-time constants.
 #pragma once
+
+#include <type_traits>
 
 #include "fft_block.hpp"
 
@@ -51,12 +53,57 @@ struct LdsPad {
     static constexpr int size(int n) { return n; }
 };
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, N).
+template <int B, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < N) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, N>(f);
+    }
+}
+
+// cos(15 a degrees), a in 0..6: the constant twiddles of sizes dividing 24.
+constexpr double kCos15[7] = {1.0,
+                              0.96592582628906828674974319972890,
+                              0.86602540378443864676372317075294,
+                              0.70710678118654752440084436210485,
+                              0.5,
+                              0.25881904510252076234889883762405,
+                              0.0};
+constexpr double cos15(int a)
+{
+    a = ((a % 24) + 24) % 24;
+    return a <= 6 ? kCos15[a] : a <= 12 ? -kCos15[12 - a] : a <= 18 ? -kCos15[a - 12] : kCos15[24 - a];
+}
+constexpr double sin15(int a) { return cos15(a - 6); }
+
+// x * exp(D i 2 pi e / N) for compile-time e and N | 24 (D = -1 forward,
+// +1 inverse): multiples of a quarter turn are exact swaps and negations,
+// the rest one complex product with constants.
+template <typename T, int D, int N, int e>
+__device__ __forceinline__ cpx<T> twiddle_const(cpx<T> x)
+{
+    static_assert(24 % N == 0, "constant twiddles cover sizes dividing 24");
+    constexpr int a = (((24 / N) * e) % 24 + 24) % 24; // the angle in 15-degree units
+    if constexpr (a == 0) {
+        return x;
+    } else if constexpr (a == 12) {
+        return mk<T>(-x.re, -x.im);
+    } else if constexpr (a == 6 || a == 18) { // W = (0, D) or (0, -D)
+        constexpr int sg = a == 6 ? D : -D;
+        return sg > 0 ? mk<T>(-x.im, x.re) : mk<T>(x.im, -x.re);
+    } else {
+        constexpr T c = (T)cos15(a), sn = (T)(D * sin15(a));
+        return mul(x, mk<T>(c, sn));
+    }
+}
+
 // In-register DFT of compile-time size N on a[0..N-1] (natural order in and
 // out). Base radices 2, 3, 4, 8; composite N = N1*N2 by Cooley-Tukey with
-// N1 the largest base radix dividing N. Internal twiddles W_N^e are read
-// from the length-n table at stride n/N.
+// N1 the largest base radix dividing N, and compile-time twiddles.
 template <typename T, int N, int D>
-__device__ __forceinline__ void dft_ct(cpx<T> *a, const cpx<T> *__restrict__ tw, int n)
+__device__ __forceinline__ void dft_ct(cpx<T> *a)
 {
     if constexpr (N == 2 || N == 3 || N == 4 || N == 8) {
         dft<T, N, D>(a);
@@ -64,24 +111,23 @@ __device__ __forceinline__ void dft_ct(cpx<T> *a, const cpx<T> *__restrict__ tw,
         constexpr int N1 = (N % 8 == 0) ? 8 : (N % 4 == 0) ? 4 : (N % 3 == 0) ? 3 : 2;
         constexpr int N2 = N / N1;
         cpx<T> y[N];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) {
+        static_for<0, N2>([&](auto n2I) {
+            constexpr int n2 = decltype(n2I)::value;
             cpx<T> b[N1];
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) b[n1] = a[N2 * n1 + n2];
             dft<T, N1, D>(b);
-#pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) {
-                if (n2 > 0 && k1 > 0) b[k1] = mul(b[k1], twiddle<T, D>(tw, (n / N) * n2 * k1));
-                y[n2 * N1 + k1] = b[k1];
-            }
-        }
+            static_for<0, N1>([&](auto k1I) {
+                constexpr int k1 = decltype(k1I)::value;
+                y[n2 * N1 + k1] = twiddle_const<T, D, N, n2 * k1>(b[k1]);
+            });
+        });
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) {
             cpx<T> c[N2];
 #pragma unroll
             for (int n2 = 0; n2 < N2; ++n2) c[n2] = y[n2 * N1 + k1];
-            dft_ct<T, N2, D>(c, tw, n);
+            dft_ct<T, N2, D>(c);
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) a[k1 + N1 * k2] = c[k2];
         }
@@ -95,20 +141,71 @@ constexpr int early_radix(int E, int rem)
     return (E % 8 == 0 && rem % 8 == 0) ? 8 : (E % 4 == 0 && rem % 4 == 0) ? 4 : 2;
 }
 
+// Number of per-lane twiddles of the early passes before the one at Ls
+// (the first pass, Ls = 1, needs none).
+template <int E, int TG, int Ls, int L = 1>
+constexpr int early_tw_offset()
+{
+    if constexpr (L >= Ls || L >= TG) {
+        return 0;
+    } else {
+        constexpr int R = early_radix(E, TG / L);
+        return (L > 1 ? R - 1 : 0) + early_tw_offset<E, TG, Ls, L * R>();
+    }
+}
+
+// The per-lane twiddles of one lane-group transform: R - 1 per early pass
+// and E - 1 for the final pass, as forward values (an inverse transform
+// multiplies by their conjugates). They are loaded in one batch, ahead of
+// the transform's LDS exchanges: a load issued after an exchange's fence
+// cannot be moved above it, so loading per pass would wait once per pass.
+template <typename T, int E, int TG>
+struct GroupTw {
+    static constexpr int n = E * TG;
+    static constexpr int NE = early_tw_offset<E, TG, TG>();
+    cpx<T> e[NE > 0 ? NE : 1];
+    cpx<T> f[E];
+
+    __device__ __forceinline__ void load(const cpx<T> *__restrict__ tw, int t)
+    {
+        load_early<1>(tw, t);
+#pragma unroll
+        for (int m = 1; m < E; ++m) f[m] = tw[m * t];
+    }
+
+  private:
+    template <int Ls>
+    __device__ __forceinline__ void load_early(const cpx<T> *__restrict__ tw, int t)
+    {
+        if constexpr (Ls < TG) {
+            constexpr int R = early_radix(E, TG / Ls);
+            if constexpr (Ls > 1) {
+                constexpr int o = early_tw_offset<E, TG, Ls>(), ts = n / (Ls * R);
+                const int k = t & (Ls - 1);
+#pragma unroll
+                for (int m = 1; m < R; ++m) e[o + m - 1] = tw[m * k * ts];
+            }
+            load_early<Ls * R>(tw, t);
+        }
+    }
+};
+
+// w for the forward direction, conj(w) for the inverse.
+template <typename T, int D>
+__device__ __forceinline__ cpx<T> dir(cpx<T> w)
+{
+    if constexpr (D > 0) w.im = -w.im;
+    return w;
+}
+
 // One early radix-R pass with Ls = product of the previous radices (Ls
 // divides TG, so k = j % Ls = t % Ls for every butterfly of the lane).
 template <typename T, int E, int TG, int R, int Ls, int D>
-__device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, int n,
-                                           const cpx<T> *__restrict__ tw)
+__device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t,
+                                           const GroupTw<T, E, TG> &g)
 {
     constexpr int Q = E / R;
     const int k = t & (Ls - 1);
-    cpx<T> w[R];
-    if constexpr (Ls > 1) {
-        const int ts = n / (Ls * R);
-#pragma unroll
-        for (int m = 1; m < R; ++m) w[m] = twiddle<T, D>(tw, m * k * ts);
-    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int j = t + TG * q;
@@ -117,8 +214,9 @@ __device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, i
 #pragma unroll
         for (int m = 0; m < R; ++m) a[m] = v[q + m * Q];
         if constexpr (Ls > 1) {
+            constexpr int o = early_tw_offset<E, TG, Ls>();
 #pragma unroll
-            for (int m = 1; m < R; ++m) a[m] = mul(a[m], w[m]);
+            for (int m = 1; m < R; ++m) a[m] = mul(a[m], dir<T, D>(g.e[o + m - 1]));
         }
         dft<T, R, D>(a);
 #pragma unroll
@@ -128,79 +226,40 @@ __device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, i
 
 template <typename T, int E, int TG, int D, int Ls>
 __device__ __forceinline__ void early_passes(cpx<T> (&v)[E], cpx<T> *lds, int t,
-                                             const cpx<T> *__restrict__ tw)
+                                             const GroupTw<T, E, TG> &g)
 {
     if constexpr (Ls < TG) {
         constexpr int R = early_radix(E, TG / Ls);
         static_assert(E % R == 0 && (TG / Ls) % R == 0, "no early radix");
-        group_pass<T, E, TG, R, Ls, D>(v, lds, t, E * TG, tw);
+        group_pass<T, E, TG, R, Ls, D>(v, lds, t, g);
         group_sync<TG>();
 #pragma unroll
         for (int s = 0; s < E; ++s) v[s] = lds[LdsPad<T>::idx(t + TG * s)];
         group_sync<TG>();
-        early_passes<T, E, TG, D, Ls * R>(v, lds, t, tw);
+        early_passes<T, E, TG, D, Ls * R>(v, lds, t, g);
     }
 }
 
 // Final radix-E pass of lane t (Ls = TG, k = t): X[t + TG*k] = sum_m
-// v[m] W_n^{m t} W_E^{m k}. With E = N1*N2 (N1 in {8, 4}) and m = N2*n1 +
-// n2 the twiddle factors as W_n^{N2 n1 t} * W_n^{n2 t}: the first goes into
-// a radix-N1 DFT over n1, the second (constant in n1) is applied after it
-// together with the Cooley-Tukey twiddle W_E^{n2 k1}, which is uniform
-// across lanes. That needs N1 - 1 + N2 - 1 per-lane twiddles, not E - 1.
+// v[m] W_n^{m t} W_E^{m k}: the E - 1 per-lane twiddles W_n^{m t}, then an
+// in-register E-point DFT whose own twiddles are compile-time constants.
 template <typename T, int E, int TG, int D>
-__device__ __forceinline__ void final_pass(cpx<T> (&v)[E], int t, const cpx<T> *__restrict__ tw)
+__device__ __forceinline__ void final_pass(cpx<T> (&v)[E], const GroupTw<T, E, TG> &g)
 {
-    constexpr int n = E * TG;
-    constexpr int N1 = (E % 8 == 0) ? 8 : (E % 4 == 0) ? 4 : 0;
-    if constexpr (N1 == 0 || E == N1) {
 #pragma unroll
-        for (int m = 1; m < E; ++m) v[m] = mul(v[m], twiddle<T, D>(tw, m * t));
-        dft_ct<T, E, D>(v, tw, n);
-    } else {
-        constexpr int N2 = E / N1;
-        cpx<T> r[N1];
-#pragma unroll
-        for (int n1 = 1; n1 < N1; ++n1) r[n1] = twiddle<T, D>(tw, N2 * n1 * t);
-        cpx<T> y[E];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) {
-            cpx<T> b[N1];
-            b[0] = v[n2];
-#pragma unroll
-            for (int n1 = 1; n1 < N1; ++n1) b[n1] = mul(v[N2 * n1 + n2], r[n1]);
-            dft<T, N1, D>(b);
-            if (n2 > 0) {
-                const cpx<T> s = twiddle<T, D>(tw, n2 * t);
-#pragma unroll
-                for (int k1 = 0; k1 < N1; ++k1) {
-                    b[k1] = mul(b[k1], s);
-                    if (k1 > 0) b[k1] = mul(b[k1], twiddle<T, D>(tw, (n / E) * n2 * k1));
-                }
-            }
-#pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) y[n2 * N1 + k1] = b[k1];
-        }
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-            cpx<T> c[N2];
-#pragma unroll
-            for (int n2 = 0; n2 < N2; ++n2) c[n2] = y[n2 * N1 + k1];
-            dft_ct<T, N2, D>(c, tw, n);
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) v[k1 + N1 * k2] = c[k2];
-        }
-    }
+    for (int m = 1; m < E; ++m) v[m] = mul(v[m], dir<T, D>(g.f[m]));
+    dft_ct<T, E, D>(v);
 }
 
 // Full transform of the group: on entry v[s] = x[t + TG*s], on exit
-// v[s] = X[t + TG*s]. lds is the group's slice (LdsPad<T>::size(n)).
+// v[s] = X[t + TG*s]. lds is the group's slice (LdsPad<T>::size(n)); g holds
+// the lane's twiddles (GroupTw::load with the table of length n).
 template <typename T, int E, int TG, int D>
 __device__ __forceinline__ void group_fft(cpx<T> (&v)[E], cpx<T> *lds, int t,
-                                          const cpx<T> *__restrict__ tw)
+                                          const GroupTw<T, E, TG> &g)
 {
-    early_passes<T, E, TG, D, 1>(v, lds, t, tw);
-    final_pass<T, E, TG, D>(v, t, tw);
+    early_passes<T, E, TG, D, 1>(v, lds, t, g);
+    final_pass<T, E, TG, D>(v, g);
 }
 
 } // namespace s2d

@@ -225,6 +225,7 @@ struct StageArgs {
     double cfl;     // > 0: dt from the CFL max of this step
     double dx;      // 2 pi / N
     double norm;    // Np * Np
+    double inv_norm; // 1 / norm
 };
 
 __device__ __forceinline__ double stage_dt(const StageArgs &a, const DevScalars *sc)

@@ -43,7 +43,9 @@ struct WaveCfg {
     static constexpr int GPB = TG >= 64 ? 1 : 64 / TG;
     static constexpr int BLK = TG >= 64 ? TG : 64; // threads per half
     static constexpr int SLICE = LdsPad<T>::size(NP);
-    static constexpr int WPE = sizeof(T) == 8 ? 3 : 4; // register budget, waves per SIMD
+    // register budget, waves per SIMD (FP64: 256 VGPRs hold a transform's
+    // data, its twiddles and the loads issued ahead of it without spills)
+    static constexpr int WPE = sizeof(T) == 8 ? 2 : 4;
     // pass B holds w and u + i v; a multi-wave group needs more registers
     static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
@@ -70,14 +72,29 @@ __device__ __forceinline__ cpx<T> sel(bool c, const cpx<T> &a, const cpx<T> &b)
     return mk<T>(c ? a.re : b.re, c ? a.im : b.im);
 }
 
-// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, N).
-template <int B, int N, typename F>
-__device__ __forceinline__ void static_for(F &&f)
+// 1/x from the hardware reciprocal and Newton steps with FMA (one in FP32,
+// two in FP64): within one ulp of the division, at a third of its cost.
+__device__ __forceinline__ float recip(float x)
 {
-    if constexpr (B < N) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, N>(f);
-    }
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+__device__ __forceinline__ double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
+// x / 3, correctly rounded, without a division: q = RN(x * RN(1/3)) is
+// within one ulp of x/3, the remainder x - 3q is exact under FMA, and one
+// correction q + r * RN(1/3) rounds to RN(x/3) (Markstein's theorem).
+template <typename T>
+__device__ __forceinline__ T div3(T x)
+{
+    constexpr T y = (T)(1.0L / 3.0L);
+    const T q = x * y;
+    return fma(fma((T)-3, q, x), y, q);
 }
 
 // Where the padded index k = t + TG*S (0 <= t < TG) of a length-NP
@@ -194,49 +211,60 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     }
     const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
     const T qx = (T)kx;
-    cpx<T> v[E];
+    // every load of the column first (unconditional, in bounds), then the
+    // twiddles, so that all of them are in flight together
+    cpx<T> x[E];
+    bool kept[E];
     static_for<0, E>([&](auto sI) {
         constexpr int s = decltype(sI)::value;
-        const int j = t + TG * s;
-        bool kept;
-        const int c = compact_row_of<NP, TG, s>(j, &kept);
-        kept = kept && active;
-        cpx<T> w = mk<T>((T)0, (T)0);
-        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
-            const cpx<T> x = src[c]; // in bounds for every lane
-            w = sel(kept, x, w);
-        }
-        const int kyi = signed_k<NP, TG, s>(j);
-        const T qy = (T)kyi;
-        if (f == 2) {
-            v[s] = w;
-        } else {
+        bool kp;
+        const int c = compact_row_of<NP, TG, s>(t + TG * s, &kp);
+        kept[s] = kp && active;
+        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) x[s] = src[c];
+    });
+    GroupTw<T, E, TG> gt;
+    gt.load(tw, t);
+    cpx<T> v[E];
+    if (f == 2) { // block-uniform
+        static_for<0, E>([&](auto sI) {
+            constexpr int s = decltype(sI)::value;
+            v[s] = mk<T>((T)0, (T)0);
+            if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) v[s] = sel(kept[s], x[s], v[s]);
+        });
+    } else {
+        // u_hat = II*ky*psi (f = 0), v_hat = -II*kx*psi (f = 1): a*II*psi
+        static_for<0, E>([&](auto sI) {
+            constexpr int s = decltype(sI)::value;
             cpx<T> ps = mk<T>((T)0, (T)0);
+            const int kyi = signed_k<NP, TG, s>(t + TG * s);
+            const T qy = (T)kyi;
             if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
+                const cpx<T> w = sel(kept[s], x[s], ps);
                 const T dd = (kx == 0) ? qy * qy : (kyi == 0) ? qx * qx : qx * qx + qy * qy;
-                const T inv = (T)1 / (dd == (T)0 ? (T)1 : dd); // one reciprocal per mode
+                const T inv = recip(dd == (T)0 ? (T)1 : dd); // one reciprocal per mode
                 ps = sel(kx == 0 && kyi == 0, ps, mk<T>(w.re * inv, w.im * inv));
             }
-            if (f == 0) {
-                v[s] = mk<T>(-(qy * ps.im), qy * ps.re); // II*ky*psi
-            } else {
-                cpx<T> r = mk<T>(-(qx * ps.im), qx * ps.re); // II*kx*psi
-                v[s] = mk<T>((T)-1 * r.re, (T)-1 * r.im);
-            }
-        }
-    });
+            const T a = (f == 0) ? qy : -qx;
+            v[s] = mk<T>(-(a * ps.im), a * ps.re);
+        });
+    }
     cpx<T> *slice = lds + g * SL;
-    group_fft<T, E, TG, 1>(v, slice, t, tw);
+    group_fft<T, E, TG, 1>(v, slice, t, gt);
     // transpose through LDS: the block's columns become LPB-element row segments
 #pragma unroll
     for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx(t + TG * s)] = v[s];
     __syncthreads();
-    cpx<T> *dst = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx0;
+    // LPB * NP = E * BLK elements: element e = threadIdx.x + BLK m is row
+    // e / LPB, column e % LPB of the tile (BLK = LPB * TG)
+    const int gi = threadIdx.x % LPB, yb = threadIdx.x / LPB;
+    if (kx0 + gi < d.K) {
+        cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx0 + gi;
+        const cpx<T> *sl = lds + gi * SL;
 #pragma unroll
-    for (int m = 0; m < E; ++m) { // LPB * NP = E * BLK elements
-        const int e = threadIdx.x + BLK * m;
-        const int y = e / LPB, gi = e % LPB;
-        if (kx0 + gi < d.K) dst[(size_t)y * d.KG + gi] = lds[gi * SL + LdsPad<T>::idx(y)];
+        for (int m = 0; m < E; ++m) {
+            const int y = yb + TG * m;
+            col[(size_t)y * d.KG] = sl[LdsPad<T>::idx(y)];
+        }
     }
 }
 
@@ -325,7 +353,9 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
         constexpr int s = decltype(sI)::value;
         v[s] = herm_load<T, NP, TG, s>(rw, t);
     });
-    group_fft<T, E, TG, 1>(v, slice, t, tw);
+    GroupTw<T, E, TG> gt; // the row's three transforms share the lane's twiddles
+    gt.load(tw, t);
+    group_fft<T, E, TG, 1>(v, slice, t, gt);
     T wr[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) wr[s] = v[s].re;
@@ -336,16 +366,19 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
         const cpx<T> a = herm_load<T, NP, TG, s>(ru, t), b = herm_load<T, NP, TG, s>(rv, t);
         v[s] = mk<T>(a.re - b.im, a.im + b.re);
     });
-    group_fft<T, E, TG, 1>(v, slice, t, tw);
+    group_fft<T, E, TG, 1>(v, slice, t, gt);
     T vmax = (T)0;
+    if (track_max) { // uniform
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const T m = absv(v[s].re) + absv(v[s].im);
-        vmax = (m > vmax || m != m) ? m : vmax; // NaN sticks
-        v[s] = mk<T>(v[s].re * wr[s], v[s].im * wr[s]); // uw + i vw
+        for (int s = 0; s < E; ++s) {
+            const T m = absv(v[s].re) + absv(v[s].im);
+            vmax = (m > vmax || m != m) ? m : vmax; // NaN sticks
+        }
     }
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = mk<T>(v[s].re * wr[s], v[s].im * wr[s]); // uw + i vw
     if (forward) {
-        group_fft<T, E, TG, -1>(v, slice, t, tw);
+        group_fft<T, E, TG, -1>(v, slice, t, gt);
 #pragma unroll
         for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx(t + TG * s)] = v[s];
         __syncthreads();
@@ -403,14 +436,44 @@ k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
 
 // ------------------------------------------------------------- pass C
 
-// The RHS, truncation and stage update of half HALF's rows (s with
-// s % 2 == HALF) of column kx; P / Q are the uw / vw column spectra (one of
-// them in registers, the other read from the partner half's LDS slice).
+// The compact-state operands of half HALF's rows (s with s % 2 == HALF) of
+// one column: the stage and step-start values and the forcing, loaded
+// before the column's transform so that their latency overlaps it.
+template <typename T, int NP, int HALF>
+struct PassCOperands {
+    static constexpr int E = WaveCfg<T, NP>::E, TG = WaveCfg<T, NP>::TG;
+    cpx<T> ws[E / 2], w0[E / 2], fv[E / 2];
+    int c[E / 2];
+    bool row[E / 2];
+
+    __device__ __forceinline__ void load(int t, const cpx<T> *Wn, const cpx<T> *Wst,
+                                         const cpx<T> *Fc, size_t col)
+    {
+        static_for<0, E / 2>([&](auto iI) {
+            constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
+            if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
+                c[i] = compact_row_of<NP, TG, s>(t + TG * s, &row[i]);
+                ws[i] = Wst[col + c[i]];
+                w0[i] = Wn[col + c[i]];
+            }
+        });
+        if (Fc) { // uniform
+            static_for<0, E / 2>([&](auto iI) {
+                constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
+                if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) fv[i] = Fc[col + c[i]];
+            });
+        }
+    }
+};
+
+// The RHS, truncation and stage update of half HALF's rows of column kx;
+// P / Q are the uw / vw column spectra (one of them in registers, the other
+// read from the partner half's LDS slice).
 template <typename T, int NP, int HALF>
 __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E],
                                              const cpx<T> *other, int t, int kx, bool active,
-                                             const cpx<T> *Wn, const cpx<T> *Wst, cpx<T> *Wout,
-                                             const cpx<T> *Fc, size_t col, const StageArgs &a,
+                                             const PassCOperands<T, NP, HALF> &op, bool forced,
+                                             cpx<T> *Wout, size_t col, const StageArgs &a,
                                              double dtd, double nu)
 {
 #pragma clang fp contract(off)
@@ -418,26 +481,8 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
     constexpr int E = C::E, TG = C::TG, H = NP / 3;
     const T dt = (T)dtd;
     const T qx = (T)kx;
-    const T inv_norm = (T)(1.0 / a.norm); // multiplications instead of the
-                                          // reference's divisions: <= 1 ulp
-    // loads first (unconditional, in bounds), so that they overlap
-    cpx<T> ws[E / 2], w0[E / 2], fv[E / 2];
-    int c[E / 2];
-    bool row[E / 2];
-    static_for<0, E / 2>([&](auto iI) {
-        constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
-        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
-            c[i] = compact_row_of<NP, TG, s>(t + TG * s, &row[i]);
-            ws[i] = Wst[col + c[i]];
-            w0[i] = Wn[col + c[i]];
-        }
-    });
-    if (Fc) { // uniform
-        static_for<0, E / 2>([&](auto iI) {
-            constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
-            if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) fv[i] = Fc[col + c[i]];
-        });
-    }
+    const T inv_norm = (T)a.inv_norm; // multiplications instead of the
+                                      // reference's divisions: <= 1 ulp
     static_for<0, E / 2>([&](auto iI) {
         constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
         if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
@@ -454,9 +499,9 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
             const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
             const cpx<T> dv = mk<T>(-(qy * Q.im), qy * Q.re); // DuDy of (vw)^
             const T sn = (T)nu * (qx * qx + qy * qy);
-            const cpx<T> wsv = ws[i], w0v = w0[i];
+            const cpx<T> wsv = op.ws[i], w0v = op.w0[i];
             cpx<T> dW = mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
-            if (Fc) dW = mk<T>(dW.re - fv[i].re, dW.im - fv[i].im); // closure / forcing
+            if (forced) dW = mk<T>(dW.re - op.fv[i].re, dW.im - op.fv[i].im); // closure / forcing
             cpx<T> out;
             if (a.stage == 1) {
                 out = mk<T>(w0v.re + dt * dW.re, w0v.im + dt * dW.im);
@@ -466,16 +511,58 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
                             ((T)0.75 * w0v.im + (T)0.25 * wsv.im) + cc * dW.im);
             } else {
                 const T cc = dt * (T)2;
-                // (w + 2 w2 + 2 dt dW)/3 with one correctly rounded division:
-                // a rounded 1/3 multiplier would bias |w| by its rounding
-                // error every step
-                out = mk<T>(((w0v.re + (T)2 * wsv.re) + cc * dW.re) / (T)3,
-                            ((w0v.im + (T)2 * wsv.im) + cc * dW.im) / (T)3);
+                // (w + 2 w2 + 2 dt dW)/3 correctly rounded: a rounded 1/3
+                // multiplier alone would bias |w| by its rounding error
+                // every step
+                out = mk<T>(div3(((w0v.re + (T)2 * wsv.re) + cc * dW.re)),
+                            div3(((w0v.im + (T)2 * wsv.im) + cc * dW.im)));
             }
             out = sel(keep, out, mk<T>((T)0, (T)0));
-            if (active && row[i]) Wout[col + c[i]] = out;
+            if (active && op.row[i]) Wout[col + op.c[i]] = out;
         }
     });
+}
+
+// One half of a pass C block: half 0 transforms uw, half 1 vw.
+template <typename T, int NP, int HALF>
+__device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
+                                           const cpx<T> *__restrict__ Hsrc, const cpx<T> *Wn,
+                                           const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
+                                           const cpx<T> *__restrict__ tw, DevScalars *sc,
+                                           const StageArgs &a, const Batch &bt)
+{
+    using C = WaveCfg<T, NP>;
+    constexpr int E = C::E, TG = C::TG;
+    const int g = lane / TG, t = lane % TG;
+    cpx<T> *mine = lds_all + (HALF * C::GPB + g) * C::SLICE;
+    const cpx<T> *other = lds_all + ((1 - HALF) * C::GPB + g) * C::SLICE;
+    const int job = blk * C::GPB + g;
+    const bool active = job < d.K * bt.nruns;
+    const int run = active ? job / d.K : 0, kx = active ? job - run * d.K : 0;
+    const cpx<T> *src = Hsrc + run * bt.sH + (size_t)kx * NP;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
+    const size_t col = run * bt.sW + (size_t)kx * d.KP;
+    const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
+    PassCOperands<T, NP, HALF> op;
+    op.load(t, Wn, Wst, Fc, col);
+    GroupTw<T, E, TG> gt;
+    gt.load(tw, t);
+    const double dtd = stage_dt(a, sc + run);
+    const double nu = bt.nu ? bt.nu[run] : a.nu;
+    group_fft<T, E, TG, -1>(v, mine, t, gt);
+    // half 0 keeps the even s and hands over the odd s, half 1 the reverse
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+        if ((s & 1) != HALF) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
+    __syncthreads();
+    passC_finish<T, NP, HALF>(v, other, t, kx, active, op, Fc != nullptr, Wout, col, a, dtd, nu);
+    if (HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0) {
+        sc[run].dt = dtd;
+        sc[run].time = sc[run].time + dtd;
+        sc[run].nadv = sc[run].nadv + 1;
+    }
 }
 
 template <typename T, int NP>
@@ -486,40 +573,13 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
                                            const cpx<T> *__restrict__ tw, DevScalars *sc,
                                            const StageArgs &a, const Batch &bt)
 {
-#pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
-    constexpr int E = C::E, TG = C::TG;
-    const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK; // half 0: uw, 1: vw
-    const int g = lane / TG, t = lane % TG;
-    cpx<T> *mine = lds_all + (half * C::GPB + g) * C::SLICE;
-    cpx<T> *other = lds_all + ((1 - half) * C::GPB + g) * C::SLICE;
-    const int job = blk * C::GPB + g;
-    const bool active = job < d.K * bt.nruns;
-    const int run = active ? job / d.K : 0, kx = active ? job - run * d.K : 0;
-    const cpx<T> *src = (half == 0 ? Huw : Hvw) + run * bt.sH + (size_t)kx * NP;
-    cpx<T> v[E];
-#pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
-    group_fft<T, E, TG, -1>(v, mine, t, tw);
-    // half 0 keeps the even s and hands over the odd s, half 1 the reverse
-#pragma unroll
-    for (int s = 0; s < E; ++s)
-        if ((s & 1) != half) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
-    __syncthreads();
-
-    const double dtd = stage_dt(a, sc + run);
-    const double nu = bt.nu ? bt.nu[run] : a.nu;
-    const size_t col = run * bt.sW + (size_t)kx * d.KP;
-    const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
-    if (half == 0) // wave-uniform
-        passC_finish<T, NP, 0>(v, other, t, kx, active, Wn, Wst, Wout, Fc, col, a, dtd, nu);
+    const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK;
+    // wave-uniform; both halves pass the same barriers
+    if (half == 0)
+        passC_half<T, NP, 0>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);
     else
-        passC_finish<T, NP, 1>(v, other, t, kx, active, Wn, Wst, Wout, Fc, col, a, dtd, nu);
-    if (a.stage == 3 && active && kx == 0 && half == 0 && t == 0) {
-        sc[run].dt = dtd;
-        sc[run].time = sc[run].time + dtd;
-        sc[run].nadv = sc[run].nadv + 1;
-    }
+        passC_half<T, NP, 1>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
 }
 
 template <typename T, int NP>

@@ -94,6 +94,9 @@ struct s2d_ctx {
     // kernel timing (s2d_profile_begin / _end)
     bool prof_on = false;
     int prof_used = 0;
+    int prof_every = 1;      // time one step in every prof_every
+    long long prof_step = 0; // steps started since s2d_profile_begin
+    bool prof_step_on = true;
     int prof_slot = -1; // event pair reserved for the next fused launch
     std::vector<hipEvent_t> prof_ev; // 2 per launch
     std::vector<int> prof_kind;

@@ -247,8 +247,11 @@ class Spectral2D:
         """SetTaylorGreen, src/SetIC.F90:70-101, into w and psi."""
         self._c(L.lib().s2d_set_taylor_green(self._h))
 
-    def profile_begin(self, max_launches: int):
+    def profile_begin(self, max_launches: int, every: int = 1):
+        """Bracket fused launches with HIP events (one step in every `every`)."""
         self._c(L.lib().s2d_profile_begin(self._h, int(max_launches)))
+        if every != 1:
+            self._c(L.lib().s2d_profile_every(self._h, int(every)))
 
     def profile_end(self):
         """Per fused kernel (pass A, B, C): (total ms, launches)."""
