@@ -156,26 +156,46 @@ constexpr int early_tw_offset()
 
 // The per-lane twiddles of one lane-group transform: R - 1 per early pass
 // and E - 1 for the final pass, as forward values (an inverse transform
-// multiplies by their conjugates). They are loaded in one batch, ahead of
-// the transform's LDS exchanges: a load issued after an exchange's fence
-// cannot be moved above it, so loading per pass would wait once per pass.
-template <typename T, int E, int TG>
+// multiplies by their conjugates). With PRE they are loaded in one batch,
+// ahead of the transform's LDS exchanges (a load issued after an exchange's
+// fence cannot be moved above it, so loading per pass waits once per pass);
+// without PRE each pass loads its own, which holds fewer registers.
+template <typename T, int E, int TG, bool PRE = true>
 struct GroupTw {
     static constexpr int n = E * TG;
     static constexpr int NE = early_tw_offset<E, TG, TG>();
-    cpx<T> e[NE > 0 ? NE : 1];
-    cpx<T> f[E];
+    cpx<T> e[PRE && NE > 0 ? NE : 1];
+    cpx<T> f[PRE ? E : 1];
+    const cpx<T> *tw;
+    int t;
 
-    __device__ __forceinline__ void load(const cpx<T> *__restrict__ tw, int t)
+    __device__ __forceinline__ void load(const cpx<T> *__restrict__ tw_, int t_)
     {
-        load_early<1>(tw, t);
+        tw = tw_;
+        t = t_;
+        if constexpr (PRE) {
+            load_early<1>();
 #pragma unroll
-        for (int m = 1; m < E; ++m) f[m] = tw[m * t];
+            for (int m = 1; m < E; ++m) f[m] = tw[m * t];
+        }
+    }
+    // twiddle m (1 <= m < R) of the radix-R early pass at Ls
+    template <int Ls, int R>
+    __device__ __forceinline__ cpx<T> early(int m) const
+    {
+        if constexpr (PRE) return e[early_tw_offset<E, TG, Ls>() + m - 1];
+        else return tw[m * (t & (Ls - 1)) * (n / (Ls * R))];
+    }
+    // twiddle m (1 <= m < E) of the final pass
+    __device__ __forceinline__ cpx<T> final(int m) const
+    {
+        if constexpr (PRE) return f[m];
+        else return tw[m * t];
     }
 
   private:
     template <int Ls>
-    __device__ __forceinline__ void load_early(const cpx<T> *__restrict__ tw, int t)
+    __device__ __forceinline__ void load_early()
     {
         if constexpr (Ls < TG) {
             constexpr int R = early_radix(E, TG / Ls);
@@ -185,7 +205,7 @@ struct GroupTw {
 #pragma unroll
                 for (int m = 1; m < R; ++m) e[o + m - 1] = tw[m * k * ts];
             }
-            load_early<Ls * R>(tw, t);
+            load_early<Ls * R>();
         }
     }
 };
@@ -200,9 +220,8 @@ __device__ __forceinline__ cpx<T> dir(cpx<T> w)
 
 // One early radix-R pass with Ls = product of the previous radices (Ls
 // divides TG, so k = j % Ls = t % Ls for every butterfly of the lane).
-template <typename T, int E, int TG, int R, int Ls, int D>
-__device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t,
-                                           const GroupTw<T, E, TG> &g)
+template <typename T, int E, int TG, int R, int Ls, int D, class G>
+__device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, const G &g)
 {
     constexpr int Q = E / R;
     const int k = t & (Ls - 1);
@@ -214,9 +233,8 @@ __device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t,
 #pragma unroll
         for (int m = 0; m < R; ++m) a[m] = v[q + m * Q];
         if constexpr (Ls > 1) {
-            constexpr int o = early_tw_offset<E, TG, Ls>();
 #pragma unroll
-            for (int m = 1; m < R; ++m) a[m] = mul(a[m], dir<T, D>(g.e[o + m - 1]));
+            for (int m = 1; m < R; ++m) a[m] = mul(a[m], dir<T, D>(g.template early<Ls, R>(m)));
         }
         dft<T, R, D>(a);
 #pragma unroll
@@ -224,39 +242,38 @@ __device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t,
     }
 }
 
-template <typename T, int E, int TG, int D, int Ls>
-__device__ __forceinline__ void early_passes(cpx<T> (&v)[E], cpx<T> *lds, int t,
-                                             const GroupTw<T, E, TG> &g)
+template <typename T, int E, int TG, int D, int Ls, class G>
+__device__ __forceinline__ void early_passes(cpx<T> (&v)[E], cpx<T> *lds, int t, const G &g)
 {
     if constexpr (Ls < TG) {
         constexpr int R = early_radix(E, TG / Ls);
         static_assert(E % R == 0 && (TG / Ls) % R == 0, "no early radix");
-        group_pass<T, E, TG, R, Ls, D>(v, lds, t, g);
+        group_pass<T, E, TG, R, Ls, D, G>(v, lds, t, g);
         group_sync<TG>();
 #pragma unroll
         for (int s = 0; s < E; ++s) v[s] = lds[LdsPad<T>::idx(t + TG * s)];
         group_sync<TG>();
-        early_passes<T, E, TG, D, Ls * R>(v, lds, t, g);
+        early_passes<T, E, TG, D, Ls * R, G>(v, lds, t, g);
     }
 }
 
 // Final radix-E pass of lane t (Ls = TG, k = t): X[t + TG*k] = sum_m
 // v[m] W_n^{m t} W_E^{m k}: the E - 1 per-lane twiddles W_n^{m t}, then an
 // in-register E-point DFT whose own twiddles are compile-time constants.
-template <typename T, int E, int TG, int D>
-__device__ __forceinline__ void final_pass(cpx<T> (&v)[E], const GroupTw<T, E, TG> &g)
+template <typename T, int E, int TG, int D, class G>
+__device__ __forceinline__ void final_pass(cpx<T> (&v)[E], const G &g)
 {
 #pragma unroll
-    for (int m = 1; m < E; ++m) v[m] = mul(v[m], dir<T, D>(g.f[m]));
+    for (int m = 1; m < E; ++m) v[m] = mul(v[m], dir<T, D>(g.final(m)));
     dft_ct<T, E, D>(v);
 }
 
 // Full transform of the group: on entry v[s] = x[t + TG*s], on exit
 // v[s] = X[t + TG*s]. lds is the group's slice (LdsPad<T>::size(n)); g holds
 // the lane's twiddles (GroupTw::load with the table of length n).
-template <typename T, int E, int TG, int D>
+template <typename T, int E, int TG, int D, bool PRE>
 __device__ __forceinline__ void group_fft(cpx<T> (&v)[E], cpx<T> *lds, int t,
-                                          const GroupTw<T, E, TG> &g)
+                                          const GroupTw<T, E, TG, PRE> &g)
 {
     early_passes<T, E, TG, D, 1>(v, lds, t, g);
     final_pass<T, E, TG, D>(v, g);

@@ -46,6 +46,10 @@ struct WaveCfg {
     // register budget, waves per SIMD (FP64: 256 VGPRs hold a transform's
     // data, its twiddles and the loads issued ahead of it without spills)
     static constexpr int WPE = sizeof(T) == 8 ? 2 : 4;
+    // pass A: FP64 loads its twiddles in one batch; FP32 per pass, which
+    // leaves registers for more waves (measured: 10 % less time for the
+    // mixed ensemble, 5 % at Np = 1536)
+    static constexpr bool TW_PRE_A = sizeof(T) == 8;
     // pass B holds w and u + i v; a multi-wave group needs more registers
     static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
@@ -222,7 +226,7 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
         kept[s] = kp && active;
         if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) x[s] = src[c];
     });
-    GroupTw<T, E, TG> gt;
+    GroupTw<T, E, TG, C::TW_PRE_A> gt;
     gt.load(tw, t);
     cpx<T> v[E];
     if (f == 2) { // block-uniform
@@ -348,24 +352,33 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
     // real transform and is dropped)
     const cpx<T> *rw = Gw + (size_t)y * d.KG, *ru = Gu + (size_t)y * d.KG,
                  *rv = Gv + (size_t)y * d.KG;
-    cpx<T> v[E];
+    // u + i v of this row (FP32 has the registers to issue its loads ahead
+    // of the w transform: measured 7 % less time for the mixed ensemble)
+    auto load_uv = [&](cpx<T>(&z)[E]) {
+        static_for<0, E>([&](auto sI) {
+            constexpr int s = decltype(sI)::value;
+            const cpx<T> a = herm_load<T, NP, TG, s>(ru, t), b = herm_load<T, NP, TG, s>(rv, t);
+            z[s] = mk<T>(a.re - b.im, a.im + b.re);
+        });
+    };
+    cpx<T> v[E], z[E];
     static_for<0, E>([&](auto sI) {
         constexpr int s = decltype(sI)::value;
         v[s] = herm_load<T, NP, TG, s>(rw, t);
     });
+    if constexpr (sizeof(T) == 4) load_uv(z);
     GroupTw<T, E, TG> gt; // the row's three transforms share the lane's twiddles
     gt.load(tw, t);
     group_fft<T, E, TG, 1>(v, slice, t, gt);
     T wr[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) wr[s] = v[s].re;
-
-    // u + i v
-    static_for<0, E>([&](auto sI) {
-        constexpr int s = decltype(sI)::value;
-        const cpx<T> a = herm_load<T, NP, TG, s>(ru, t), b = herm_load<T, NP, TG, s>(rv, t);
-        v[s] = mk<T>(a.re - b.im, a.im + b.re);
-    });
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = z[s];
+    } else {
+        load_uv(v);
+    }
     group_fft<T, E, TG, 1>(v, slice, t, gt);
     T vmax = (T)0;
     if (track_max) { // uniform
