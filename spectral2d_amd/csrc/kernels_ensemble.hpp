@@ -7,9 +7,9 @@
 // blockIdx % 8, is also its XCD within the class), and each block runs the
 // pass body of its class's compiled size. The sizes served are those whose
 // lane group fits one wave (Np / E <= 64): every class then uses the same
-// blocks, EnsCfg::BLK_A / BLK_B lanes for passes A and B (BLK / TG lines)
-// and 128 for pass C, and every block carries the same number of points
-// whatever its class.
+// blocks: BLK_A lanes for pass A (BLK_A / TG columns), BLK_B for pass B
+// (BLK_B / TG rows, passB_single) and 128 for pass C, and every block
+// carries the same number of points whatever its class.
 #pragma once
 
 #include "kernels_fused_v2.hpp"
@@ -104,7 +104,7 @@ k3_ens_passB(EnsArgs<T> ea, int stage)
     int b = blockIdx.x;
     const EnsClass<T> &c = ea.c[ens_class(ea, 1, b)];
     S2D_ENS_SIZES(T, c.np,
-                  (passB_body<T, NP, EnsCfg<T>::lpb_b(NP)>(b, c.nblk[1], lds, c.G0, c.G1, c.G2,
+                  (passB_single<T, NP, EnsCfg<T>::lpb_b(NP)>(b, c.nblk[1], lds, c.G0, c.G1, c.G2,
                                                           c.H0, c.H1, c.d, c.tw_row, c.sc, 1,
                                                           stage == 1, c.bt)))
 }

@@ -12,10 +12,12 @@
 //     inverse transform along y. A block holds LPB adjacent columns of one
 //     field; it stages them in LDS and stores the G rows (G[y*KG + kx]) as
 //     LPB-element contiguous segments.
-//  B  one lane group per row y (contiguous loads of its G rows): w from its
-//     Hermitian spectrum, u + i v (two-for-one), the products (u w + i v w)
-//     in registers, one forward transform, the uw / vw spectra separated
-//     from P_k and conj P_{Np-k}. A block holds LPB adjacent rows and stores
+//  B  one lane group per row y, or per two rows from Np = 1536 up
+//     (contiguous loads of the G rows): w from its Hermitian spectrum (two
+//     rows' w from one transform of w_a + i w_b), u + i v (two-for-one), the
+//     products (u w + i v w) in registers, one forward transform, the uw /
+//     vw spectra separated from P_k and conj P_{Np-k}. A block holds LPB
+//     adjacent rows and stores
 //     the H columns (H[kx*Np + y]) as LPB-element segments through LDS.
 //     Blocks are assigned XCD-major so that the tiles in flight on one XCD
 //     are adjacent and share the lines of the (kx, y) fields in its L2.
@@ -38,7 +40,10 @@ struct WaveCfg {
     static constexpr int TG = NP / E;
     // lines (columns / rows) per block of passes A and B, LPB * TG threads
     static constexpr int LPB_A = TG >= 256 ? (sizeof(T) == 4 ? 2 : 1) : 256 / TG;
-    static constexpr int LPB_B = TG >= 128 && sizeof(T) == 8 ? 1 : TG >= 256 ? 2 : 256 / TG;
+    // pass B: two rows per lane group from Np = 1536 up (passB_pair)
+    static constexpr bool PAIR_B = NP >= 1536;
+    static constexpr int LPB_B = PAIR_B ? 2 : TG >= 128 && sizeof(T) == 8 ? 1 : TG >= 256 ? 2 : 256 / TG;
+    static constexpr int THR_B = PAIR_B ? LPB_B * TG / 2 : LPB_B * TG; // lanes per block
     // pass C: column groups per half, two halves per block
     static constexpr int GPB = TG >= 64 ? 1 : 64 / TG;
     static constexpr int BLK = TG >= 64 ? TG : 64; // threads per half
@@ -50,6 +55,9 @@ struct WaveCfg {
     // leaves registers for more waves (measured: 10 % less time for the
     // mixed ensemble, 5 % at Np = 1536)
     static constexpr bool TW_PRE_A = sizeof(T) == 8;
+    // pass B: FP32 holds the twiddles of its five transforms in registers;
+    // FP64 has no room for them next to the data (256 VGPRs)
+    static constexpr bool TW_PRE_B = sizeof(T) == 4;
     // pass B holds w and u + i v; a multi-wave group needs more registers
     static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
@@ -322,9 +330,14 @@ __device__ __forceinline__ cpx<T> herm_load(const cpx<T> *__restrict__ row, int 
     }
 }
 
-// Jobs: (run, tile of LPB adjacent rows).
+// Jobs: (run, tile of LPB adjacent rows). Two bodies: passB_single (a lane
+// group per row, LPB * TG lanes) and passB_pair (a lane group per two rows,
+// LPB * TG / 2 lanes, five transforms per two rows instead of six). The
+// single-row tiles keep more waves in flight where blocks are many and
+// short (the mixed ensemble); the pairs win where the transform is long
+// (Np >= 1536).
 template <typename T, int NP, int LPB>
-__device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
+__device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Gu,
                                            const cpx<T> *__restrict__ Gv,
                                            const cpx<T> *__restrict__ Gw,
@@ -336,7 +349,7 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
 #pragma clang fp contract(off)
     using C = WaveCfg<T, NP>;
     using Tl = TileCfg<T, NP, LPB>;
-    constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL;
+    constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL; // BLK lanes
     constexpr int tiles = NP / LPB;
     static_assert(NP % LPB == 0, "row tiles must divide the rows");
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
@@ -428,13 +441,136 @@ __device__ __forceinline__ void passB_body(int blk, int nblk, cpx<T> *lds,
 }
 
 template <typename T, int NP, int LPB>
+__device__ __forceinline__ void passB_pair(int blk, int nblk, cpx<T> *lds,
+                                           const cpx<T> *__restrict__ Gu,
+                                           const cpx<T> *__restrict__ Gv,
+                                           const cpx<T> *__restrict__ Gw,
+                                           cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
+                                           const FusedDims &d, const cpx<T> *__restrict__ tw,
+                                           DevScalars *sc, int forward, int track_max,
+                                           const Batch &bt)
+{
+#pragma clang fp contract(off)
+    using C = WaveCfg<T, NP>;
+    using Tl = TileCfg<T, NP, LPB>;
+    constexpr int E = C::E, TG = C::TG, THR = Tl::BLK / 2, SL = Tl::SL;
+    constexpr int tiles = NP / LPB;
+    static_assert(NP % LPB == 0 && LPB % 2 == 0, "row tiles: even, dividing the rows");
+    const int g = threadIdx.x / TG, t = threadIdx.x % TG;
+    const int job = xcd_major(blk, nblk);
+    if (job >= tiles * bt.nruns) return; // the whole block
+    // lane group g takes the rows ya = y0 + 2g and ya + 1
+    const int run = job / tiles, y0 = (job - run * tiles) * LPB, ya = y0 + 2 * g;
+    cpx<T> *sla = lds + (2 * g) * SL, *slb = sla + SL;
+    Gu += run * bt.sG;
+    Gv += run * bt.sG;
+    Gw += run * bt.sG;
+    const size_t oa = (size_t)ya * d.KG, ob = oa + d.KG;
+
+    // u + i v of a row from its two Hermitian spectra (two-for-one)
+    auto load_uv = [&](cpx<T>(&z)[E], size_t o) {
+        static_for<0, E>([&](auto sI) {
+            constexpr int s = decltype(sI)::value;
+            const cpx<T> a = herm_load<T, NP, TG, s>(Gu + o, t);
+            const cpx<T> b = herm_load<T, NP, TG, s>(Gv + o, t);
+            z[s] = mk<T>(a.re - b.im, a.im + b.re);
+        });
+    };
+    // w of both rows from one transform: w_a + i w_b
+    cpx<T> v[E], z[E];
+    static_for<0, E>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        const cpx<T> a = herm_load<T, NP, TG, s>(Gw + oa, t);
+        const cpx<T> b = herm_load<T, NP, TG, s>(Gw + ob, t);
+        v[s] = mk<T>(a.re - b.im, a.im + b.re);
+    });
+    // FP32 has the registers to issue row a's u, v loads ahead of the w
+    // transform
+    if constexpr (sizeof(T) == 4) load_uv(z, oa);
+    GroupTw<T, E, TG, C::TW_PRE_B> gt; // the lane's twiddles, shared by the five transforms
+    gt.load(tw, t);
+    group_fft<T, E, TG, 1>(v, sla, t, gt);
+    // w_a stays in registers; w_b waits in row b's slice (unused until row
+    // b), each lane reading back only what it wrote
+    T wa[E];
+    T *wb_lds = reinterpret_cast<T *>(slb);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        wa[s] = v[s].re;
+        wb_lds[t + TG * s] = v[s].im;
+    }
+    T vmax = (T)0;
+    // one row: u + i v, the products (u w + i v w), their forward transform
+    // into the row's LDS slice
+    auto row = [&](cpx<T>(&x)[E], const T(&w)[E], cpx<T> *sl) {
+        group_fft<T, E, TG, 1>(x, sl, t, gt);
+        if (track_max) { // uniform
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const T m = absv(x[s].re) + absv(x[s].im);
+                vmax = (m > vmax || m != m) ? m : vmax; // NaN sticks
+            }
+        }
+        if (forward) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) x[s] = mk<T>(x[s].re * w[s], x[s].im * w[s]);
+            group_fft<T, E, TG, -1>(x, sl, t, gt);
+#pragma unroll
+            for (int s = 0; s < E; ++s) sl[LdsPad<T>::idx(t + TG * s)] = x[s];
+        }
+    };
+    if constexpr (sizeof(T) != 4) load_uv(z, oa);
+    row(z, wa, sla);
+    load_uv(z, ob);
+    T wb[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) wb[s] = wb_lds[t + TG * s];
+    // row b's transform writes anywhere in the slice: in a multi-wave group
+    // every wave must have read its w_b first
+    group_sync<TG>();
+    row(z, wb, slb);
+    if (forward) {
+        __syncthreads();
+        // k <= N/2 = NP/3 of the block's LPB rows, rows fastest: LPB-element
+        // segments of the H columns
+        constexpr int H = NP / 3, CNT = (H + 1) * LPB;
+        Huw += run * bt.sH + y0;
+        Hvw += run * bt.sH + y0;
+#pragma unroll 1
+        for (int m = 0; m < (CNT + THR - 1) / THR; ++m) {
+            const int e = threadIdx.x + THR * m;
+            if (e < CNT) {
+                const int k = e / LPB, r = e % LPB;
+                const cpx<T> *sl = lds + r * SL;
+                const cpx<T> P = sl[LdsPad<T>::idx(k)];
+                cpx<T> Pc = sl[LdsPad<T>::idx((NP - k) % NP)];
+                Pc.im = -Pc.im;
+                const cpx<T> su = add(P, Pc), df = sub(P, Pc);
+                Huw[(size_t)k * NP + r] = mk<T>((T)0.5 * su.re, (T)0.5 * su.im);
+                Hvw[(size_t)k * NP + r] = mk<T>((T)0.5 * df.im, (T)-0.5 * df.re);
+            }
+        }
+    }
+    if (!track_max) return;
+    double dm = (double)vmax;
+    if (dm != dm) dm = __longlong_as_double(0x7ff8000000000000ll);
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(dm, off);
+        dm = (o > dm || o != o) ? o : dm;
+    }
+    // every row of a block belongs to the same run (LPB divides NP)
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
+}
+
+template <typename T, int NP, int LPB>
 __host__ __device__ constexpr int passB_blocks(int nruns)
 {
     return NP / LPB * nruns;
 }
 
 template <typename T, int NP>
-__global__ __launch_bounds__((TileCfg<T, NP, WaveCfg<T, NP>::LPB_B>::BLK))
+__global__ __launch_bounds__((WaveCfg<T, NP>::THR_B))
 __attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void
 k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
          const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
@@ -443,8 +579,12 @@ k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
 {
     constexpr int LPB = WaveCfg<T, NP>::LPB_B;
     __shared__ cpx<T> lds[TileCfg<T, NP, LPB>::LDS];
-    passB_body<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc, forward,
-                           track_max, bt);
+    if constexpr (WaveCfg<T, NP>::PAIR_B)
+        passB_pair<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc,
+                               forward, track_max, bt);
+    else
+        passB_single<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc,
+                                 forward, track_max, bt);
 }
 
 // ------------------------------------------------------------- pass C

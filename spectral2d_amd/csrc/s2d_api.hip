@@ -657,7 +657,7 @@ int launch2_passB(s2d_ctx *c, int forward, int track_max)
     FusedDims d = fdims(c);
     int blocks = passB_blocks<T, NP, LPB>(c->nruns);
     blocks = (blocks + 7) / 8 * 8;
-    launch(c, k2_passB<T, NP>, dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0,
+    launch(c, k2_passB<T, NP>, dim3(blocks), dim3(WaveCfg<T, NP>::THR_B), 0,
            (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1],
                        (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d,
                        (const cpx<T> *)c->rowf.tw, c->d_sc, forward, track_max, batch_of(c));
