@@ -82,7 +82,7 @@ __device__ __forceinline__ int ens_class(const EnsArgs<T> &ea, int p, int &b)
 
 template <typename T>
 __global__ __launch_bounds__(EnsCfg<T>::BLK_A)
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192>::WPE))) void
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192>::WPE_A))) void
 k3_ens_passA(EnsArgs<T> ea, int stage)
 {
     __shared__ cpx<T> lds[EnsCfg<T>::LDS_A];
@@ -111,7 +111,7 @@ k3_ens_passB(EnsArgs<T> ea, int stage)
 
 template <typename T>
 __global__ __launch_bounds__(128)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : WaveCfg<T, 192>::WPE))) void
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192>::WPE_C))) void
 k3_ens_passC(EnsArgs<T> ea, int stage)
 {
     __shared__ cpx<T> lds[EnsCfg<T>::LDS_C];

@@ -48,9 +48,15 @@ struct WaveCfg {
     static constexpr int GPB = TG >= 64 ? 1 : 64 / TG;
     static constexpr int BLK = TG >= 64 ? TG : 64; // threads per half
     static constexpr int SLICE = LdsPad<T>::size(NP);
-    // register budget, waves per SIMD (FP64: 256 VGPRs hold a transform's
-    // data, its twiddles and the loads issued ahead of it without spills)
-    static constexpr int WPE = sizeof(T) == 8 ? 2 : 4;
+    // register budgets, waves per SIMD, of passes A and C (FP64: 256 VGPRs
+    // hold a transform's data, its twiddles and the loads issued ahead of
+    // it without spills)
+    static constexpr int WPE_A = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int WPE_C = sizeof(T) == 8 ? 2 : 4;
+    // pass C: twiddles in one batch, and the compact-state operands loaded
+    // ahead of the transform
+    static constexpr bool TW_PRE_C = true;
+    static constexpr bool OPS_EARLY_C = true;
     // pass A: FP64 loads its twiddles in one batch; FP32 per pass, which
     // leaves registers for more waves (measured: 10 % less time for the
     // mixed ensemble, 5 % at Np = 1536)
@@ -289,7 +295,7 @@ __host__ __device__ constexpr int passA_blocks(int K, int nruns)
 
 template <typename T, int NP>
 __global__ __launch_bounds__((TileCfg<T, NP, WaveCfg<T, NP>::LPB_A>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_A))) void
 k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
          cpx<T> *__restrict__ Gw, FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc,
          int reset_max, Batch bt)
@@ -699,12 +705,13 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     const size_t col = run * bt.sW + (size_t)kx * d.KP;
     const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
     PassCOperands<T, NP, HALF> op;
-    op.load(t, Wn, Wst, Fc, col);
-    GroupTw<T, E, TG> gt;
+    if constexpr (C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
+    GroupTw<T, E, TG, C::TW_PRE_C> gt;
     gt.load(tw, t);
     const double dtd = stage_dt(a, sc + run);
     const double nu = bt.nu ? bt.nu[run] : a.nu;
     group_fft<T, E, TG, -1>(v, mine, t, gt);
+    if constexpr (!C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
     // half 0 keeps the even s and hands over the odd s, half 1 the reverse
 #pragma unroll
     for (int s = 0; s < E; ++s)
@@ -737,7 +744,7 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
 
 template <typename T, int NP>
 __global__ __launch_bounds__((2 * WaveCfg<T, NP>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE))) void
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_C))) void
 k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
          const cpx<T> *Wst, cpx<T> *Wout, FusedDims d, const cpx<T> *__restrict__ tw,
          const int *__restrict__ trunc_i1, DevScalars *sc, StageArgs a, Batch bt)
