@@ -25,6 +25,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "s2d.h"
@@ -38,7 +39,6 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 constexpr int CH = 64;       // channels of the hidden layers
 constexpr int NMID = 4;      // 64 -> 64 layers
 constexpr int TPX = 128;     // pixels per weight-gradient tile (one row segment)
-constexpr int CMT = 1;       // M tiles per wave of the forward / data conv (64 CMT pixels)
 constexpr int LP = CH + 8;   // LDS pitch (BF16) of a staged pixel
 constexpr int W64 = 9 * CH * CH;
 
@@ -80,53 +80,80 @@ __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - 
 
 // ------------------------------------------------------------ layer 0
 // Input planes: channel c of sample n at xc + n*sN, row pitch ld (floats).
+// Grid (W / 32, B * H): a block is 32 pixels of one row x 8 groups of 8
+// output channels.
 __global__ __launch_bounds__(256) void k_conv_in(const float *__restrict__ x0,
                                                  const float *__restrict__ x1, long long sN,
                                                  int ld, float s0, float s1,
                                                  const float *__restrict__ P, u16 *__restrict__ z,
                                                  u16 *__restrict__ a, int B, int H, int W)
 {
-    __shared__ float w[9 * 2 * CH + CH];
-    for (int i = threadIdx.x; i < 9 * 2 * CH; i += blockDim.x) w[i] = bfr(P[OFF_W0 + i]);
-    for (int i = threadIdx.x; i < CH; i += blockDim.x) w[9 * 2 * CH + i] = P[OFF_B0 + i];
-    __syncthreads();
-    const long long npx = (long long)B * H * W;
-    S2D_GRID_STRIDE(q, npx * 8) // (pixel, group of 8 output channels)
-    {
-        const long long p = q >> 3;
-        const int c8 = 8 * (int)(q & 7);
-        const int xx = (int)(p % W), yy = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
-        float acc[8];
-        for (int k = 0; k < 8; ++k) acc[k] = w[9 * 2 * CH + c8 + k];
-        for (int t = 0; t < 9; ++t) {
-            const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
-            const long long o = n * sN + (long long)y2 * ld + x2;
-            const float i0 = bfr(s0 * x0[o]), i1 = bfr(s1 * x1[o]);
-            for (int k = 0; k < 8; ++k)
-                acc[k] += i0 * w[(t * 2 + 0) * CH + c8 + k] + i1 * w[(t * 2 + 1) * CH + c8 + k];
-        }
-        u16 zo[8], ao[8];
-        for (int k = 0; k < 8; ++k) {
-            zo[k] = f2bf(acc[k]);
-            ao[k] = f2bf(gelu(acc[k]));
-        }
-        *reinterpret_cast<uint4 *>(z + p * CH + c8) = *reinterpret_cast<const uint4 *>(zo);
-        *reinterpret_cast<uint4 *>(a + p * CH + c8) = *reinterpret_cast<const uint4 *>(ao);
+    constexpr int NW = 9 * 2 * CH + CH, NWI = (NW + 255) / 256;
+    __shared__ float w[NW];
+    const int row = blockIdx.y, n = row / H, yy = row - n * H; // uniform
+    const int xx = blockIdx.x * 32 + (threadIdx.x >> 3), c8 = 8 * (threadIdx.x & 7);
+    // every load first (weights and the 18 inputs), then the arithmetic
+    float wv[NWI], i0[9], i1[9];
+#pragma unroll
+    for (int i = 0; i < NWI; ++i) {
+        const int j = threadIdx.x + 256 * i;
+        if (j < NW) wv[i] = j < 9 * 2 * CH ? P[OFF_W0 + j] : P[OFF_B0 + j - 9 * 2 * CH];
     }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
+        const long long o = n * sN + (long long)y2 * ld + x2;
+        i0[t] = x0[o];
+        i1[t] = x1[o];
+    }
+#pragma unroll
+    for (int i = 0; i < NWI; ++i) {
+        const int j = threadIdx.x + 256 * i;
+        if (j < NW) w[j] = j < 9 * 2 * CH ? bfr(wv[i]) : wv[i];
+    }
+    __syncthreads();
+    float acc[8];
+    for (int k = 0; k < 8; ++k) acc[k] = w[9 * 2 * CH + c8 + k];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const float a0 = bfr(s0 * i0[t]), a1 = bfr(s1 * i1[t]);
+        for (int k = 0; k < 8; ++k)
+            acc[k] += a0 * w[(t * 2 + 0) * CH + c8 + k] + a1 * w[(t * 2 + 1) * CH + c8 + k];
+    }
+    u16 zo[8], ao[8];
+    for (int k = 0; k < 8; ++k) {
+        zo[k] = f2bf(acc[k]);
+        ao[k] = f2bf(gelu(acc[k]));
+    }
+    const size_t p = ((size_t)row * W + xx) * CH + c8;
+    *reinterpret_cast<uint4 *>(z + p) = *reinterpret_cast<const uint4 *>(zo);
+    *reinterpret_cast<uint4 *>(a + p) = *reinterpret_cast<const uint4 *>(ao);
 }
 
 // ------------------------------------------------- layers 1..4 (MFMA)
-// One block: a segment of CT = 32 MT pixels of one row x 64 output
-// channels; wave w computes pixels 16 MT w .. (one or two 16-pixel M tiles)
-// x 4 N tiles of 16 channels. K = 9 taps x 64 input channels, in chunks of 16. The input rows
-// y-1..y+1 (with a one-pixel periodic halo) are staged in LDS, the weights
-// tap by tap (double-buffered).
+// One block: a segment of CT = 64 MT pixels of one row x 64 output
+// channels; wave w computes pixels 16 MT w .. (MT 16-pixel M tiles) x 4 N
+// tiles of 16 channels. K = 9 taps x 64 input channels, in chunks of 16. The
+// input rows y-1..y+1 (with a one-pixel periodic halo) are staged in LDS.
+// Every thread issues its share of all nine taps' weights at the start (18
+// 16-byte loads in flight with the input staging: one memory latency per
+// block), then copies tap by tap into a double-buffered LDS tile.
 //   mode 0: out = acc + bias -> zo (BF16), gelu(acc + bias) -> ao (BF16)
 //   mode 1: out = acc * gelu'(zp) -> zo (BF16)   (the backward data pass:
 //           in = dZ_l, wt = the flipped transpose of W_l, zp = z_{l-1})
 // wt: BF16 [9][64 co][64 ci].
+// f(std::integral_constant<int, i>) for i = I..N-1, unrolled at compile time.
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void for_each_c(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        for_each_c<N, I + 1>(f);
+    }
+}
+
 template <int MT>
-__global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv64(const u16 *__restrict__ in,
                                                 const u16 *__restrict__ wt,
                                                 const float *__restrict__ bias,
                                                 const u16 *__restrict__ zp, u16 *__restrict__ zo,
@@ -142,30 +169,52 @@ __global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
     const int y = b % H, n = b / H;
     const int x0 = seg * CT;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int q = tid; q < 3 * (CT + 2) * 8; q += 256) {
-        const int chunk = q & 7, pr = q >> 3;
-        const int r = pr / (CT + 2), px = pr % (CT + 2);
-        const int yy = wrap(y + r - 1, H), xx = wrap(x0 + px - 1, W);
-        const uint4 v =
-            *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CH + 8 * chunk);
-        *reinterpret_cast<uint4 *>(tin + pr * LP + 8 * chunk) = v;
-    }
-    auto load_w = [&](int t, int buf) {
-        for (int q = tid; q < CH * 8; q += 256) {
-            const int co = q >> 3, chunk = q & 7;
-            *reinterpret_cast<uint4 *>(tw[buf] + co * LP + 8 * chunk) =
-                *reinterpret_cast<const uint4 *>(wt + ((size_t)t * CH + co) * CH + 8 * chunk);
+    uint4 wr[9][2]; // this thread's 16-byte chunks of every tap: q = tid + 256 i
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = tid + 256 * i, co = q >> 3, chunk = q & 7;
+            wr[t][i] = *reinterpret_cast<const uint4 *>(wt + ((size_t)t * CH + co) * CH + 8 * chunk);
         }
+    // the input rows: all loads of the thread in flight together, then the
+    // LDS stores
+    constexpr int NST = 3 * (CT + 2) * 8, NSI = (NST + 255) / 256;
+    uint4 st[NSI];
+    for_each_c<NSI>([&](auto iI) {
+        constexpr int i = decltype(iI)::value;
+        const int q = tid + 256 * i;
+        st[i] = uint4{0, 0, 0, 0};
+        if (q < NST) {
+            const int chunk = q & 7, pr = q >> 3;
+            const int r = pr / (CT + 2), px = pr % (CT + 2);
+            const int yy = wrap(y + r - 1, H), xx = wrap(x0 + px - 1, W);
+            st[i] = *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CH +
+                                                     8 * chunk);
+        }
+    });
+    for_each_c<NSI>([&](auto iI) {
+        constexpr int i = decltype(iI)::value;
+        const int q = tid + 256 * i;
+        if (q < NST) *reinterpret_cast<uint4 *>(tin + (q >> 3) * LP + 8 * (q & 7)) = st[i];
+    });
+    // thread tid holds chunks q = tid and tid + 256: co = q >> 3 (and co + 32)
+    const int wco = tid >> 3, wch = tid & 7;
+    auto store_w = [&](uint4 lo, uint4 hi, int buf) {
+        *reinterpret_cast<uint4 *>(tw[buf] + wco * LP + 8 * wch) = lo;
+        *reinterpret_cast<uint4 *>(tw[buf] + (wco + 32) * LP + 8 * wch) = hi;
     };
-    load_w(0, 0);
+    store_w(wr[0][0], wr[0][1], 0);
     __syncthreads();
     float4_t acc[MT][4];
     for (int m = 0; m < MT; ++m)
         for (int j = 0; j < 4; ++j) acc[m][j] = float4_t{0.f, 0.f, 0.f, 0.f};
     const int row16 = lane & 15, kq = 4 * (lane >> 4);
-    for (int t = 0; t < 9; ++t) {
-        if (t + 1 < 9) load_w(t + 1, (t + 1) & 1);
-        const int dy = t / 3, dx = t % 3;
+    // taps at compile time, so that wr stays in registers
+    for_each_c<9>([&](auto tI) {
+        constexpr int t = decltype(tI)::value;
+        if constexpr (t + 1 < 9) store_w(wr[t + 1][0], wr[t + 1][1], (t + 1) & 1);
+        constexpr int dy = t / 3, dx = t % 3;
         const u16 *wb = tw[t & 1];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -187,7 +236,7 @@ __global__ __launch_bounds__(256) void k_conv64(const u16 *__restrict__ in,
                                                                           0, 0, 0);
         }
         __syncthreads();
-    }
+    });
     // results through LDS (the input tile is free): [px][co] FP32, pitch 68
     constexpr int SOP = CH + 4;
     float *so = reinterpret_cast<float *>(tin);
@@ -320,34 +369,44 @@ __global__ __launch_bounds__(256) void k_dw64(const u16 *__restrict__ dz,
 // ------------------------------------------------------------ layer 5
 // y[p] = b5 + sum_t sum_ci W5[t][ci] a[p + off_t][ci]  (FP32 output, dense
 // [B][H][W]).
+// Grid (W / 32, B * H): 8 lanes per pixel, 8 channels each.
 __global__ __launch_bounds__(256) void k_conv_out(const u16 *__restrict__ a,
                                                   const float *__restrict__ P,
                                                   float *__restrict__ y, int B, int H, int W)
 {
-    __shared__ float w[9 * CH + 1];
-    for (int i = threadIdx.x; i < 9 * CH; i += blockDim.x) w[i] = bfr(P[OFF_W5 + i]);
-    if (threadIdx.x == 0) w[9 * CH] = P[OFF_B5];
-    __syncthreads();
-    const long long npx = (long long)B * H * W;
-    // 8 lanes per pixel, 8 channels each; grid-stride over whole groups so
-    // that every lane of a group takes part in the shuffles
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < npx * 8;
-         q += (long long)gridDim.x * blockDim.x) {
-        const long long p = q >> 3;
-        const int c8 = 8 * (int)(q & 7);
-        const int xx = (int)(p % W), yy = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
-        float s = 0.f;
-        for (int t = 0; t < 9; ++t) {
-            const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
-            const uint4 v = *reinterpret_cast<const uint4 *>(a + (((size_t)n * H + y2) * W + x2) * CH + c8);
-            const u16 *h = reinterpret_cast<const u16 *>(&v);
-            for (int k = 0; k < 8; ++k) s += bf2f(h[k]) * w[t * CH + c8 + k];
-        }
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        s += __shfl_xor(s, 4);
-        if ((q & 7) == 0) y[p] = s + w[9 * CH];
+    constexpr int NW = 9 * CH + 1, NWI = (NW + 255) / 256;
+    __shared__ float w[NW];
+    const int row = blockIdx.y, n = row / H, yy = row - n * H; // uniform
+    const int xx = blockIdx.x * 32 + (threadIdx.x >> 3), c8 = 8 * (threadIdx.x & 7);
+    // every load first (weights and the nine input chunks), then the arithmetic
+    float wv[NWI];
+    uint4 v[9];
+#pragma unroll
+    for (int i = 0; i < NWI; ++i) {
+        const int j = threadIdx.x + 256 * i;
+        if (j < NW) wv[i] = P[OFF_W5 + j]; // OFF_B5 = OFF_W5 + 9 CH
     }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
+        v[t] = *reinterpret_cast<const uint4 *>(a + (((size_t)n * H + y2) * W + x2) * CH + c8);
+    }
+#pragma unroll
+    for (int i = 0; i < NWI; ++i) {
+        const int j = threadIdx.x + 256 * i;
+        if (j < NW) w[j] = j < 9 * CH ? bfr(wv[i]) : wv[i];
+    }
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const u16 *h = reinterpret_cast<const u16 *>(&v[t]);
+        for (int k = 0; k < 8; ++k) s += bf2f(h[k]) * w[t * CH + c8 + k];
+    }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 4);
+    if ((threadIdx.x & 7) == 0) y[(size_t)row * W + xx] = s + w[9 * CH];
 }
 
 // Scaled output planes: out[n][yy][xx] (row pitch ld, sample stride sN) =
@@ -358,14 +417,12 @@ __global__ void k_write_out(const float *__restrict__ y, float *__restrict__ out
                             int ld, float scale, int B, int H, int W,
                             const float *__restrict__ w, long long wN, int wld, int clip)
 {
-    const long long npx = (long long)B * H * W;
-    S2D_GRID_STRIDE(p, npx)
-    {
-        const int xx = (int)(p % W), yy = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
-        float v = scale * y[p];
-        if (clip && !(v * w[n * wN + (long long)yy * wld + xx] > 0.f)) v = 0.f;
-        out[n * sN + (long long)yy * ld + xx] = v;
-    }
+    // grid (W / 128, B * H), 128 pixels of one row per block
+    const int row = blockIdx.y, n = row / H, yy = row - n * H; // uniform
+    const int xx = blockIdx.x * 128 + threadIdx.x;
+    float v = scale * y[(size_t)row * W + xx];
+    if (clip && !(v * w[n * wN + (long long)yy * wld + xx] > 0.f)) v = 0.f;
+    out[n * sN + (long long)yy * ld + xx] = v;
 }
 
 // Loss (mean square of y - target / out_scale) partial sums and dy.
@@ -610,21 +667,30 @@ unsigned grid_for(long long n, int blk)
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// A 64 -> 64 layer in 128-pixel row segments (W is a multiple of 128): at
+// 256^2 that is 512 blocks, two per CU, all resident at once.
+void conv64(s2d_cnn *c, int B, const u16 *in, const u16 *wt, const float *bias, const u16 *zp,
+            u16 *zo, u16 *ao, int mode)
+{
+    const int H = c->H, W = c->W;
+    hipLaunchKernelGGL(k_conv64<2>, dim3(B * H * (W / 128)), dim3(256), 0, c->stream, in, wt,
+                       bias, zp, zo, ao, H, W, mode);
+}
+
 // Forward of B samples into c->y (raw network output).
 int forward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, int ld)
 {
     const int H = c->H, W = c->W;
     const long long npx = (long long)B * H * W;
-    hipLaunchKernelGGL(k_conv_in, dim3(grid_for(npx * 8, 256)), dim3(256), 0, c->stream, x0, x1, sN,
+    hipLaunchKernelGGL(k_conv_in, dim3(W / 32, B * H), dim3(256), 0, c->stream, x0, x1, sN,
                        ld, c->in_scale[0], c->in_scale[1], c->P, c->z[0], c->a[0], B, H, W);
     CNN_TRY(launched(c, "k_conv_in"));
     for (int l = 1; l <= NMID; ++l) {
-        hipLaunchKernelGGL(k_conv64<CMT>, dim3(B * H * (W / (64 * CMT))), dim3(256), 0, c->stream,
-                           c->a[l - 1], c->wf + (size_t)(l - 1) * W64, c->P + off_bl(l),
-                           (const u16 *)nullptr, c->z[l], c->a[l], H, W, 0);
+        conv64(c, B, c->a[l - 1], c->wf + (size_t)(l - 1) * W64, c->P + off_bl(l),
+               (const u16 *)nullptr, c->z[l], c->a[l], 0);
         CNN_TRY(launched(c, "k_conv64"));
     }
-    hipLaunchKernelGGL(k_conv_out, dim3(grid_for(npx * 8, 256)), dim3(256), 0, c->stream, c->a[NMID],
+    hipLaunchKernelGGL(k_conv_out, dim3(W / 32, B * H), dim3(256), 0, c->stream, c->a[NMID],
                        c->P, c->y, B, H, W);
     return launched(c, "k_conv_out");
 }
@@ -658,10 +724,8 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
         hipLaunchKernelGGL(k_reduce, dim3(grid_for(W64 + CH, 256)), dim3(256), 0, c->stream,
                            c->part, ndw, W64 + CH, c->G + off_wl(l));
         CNN_TRY(launched(c, "k_reduce"));
-        hipLaunchKernelGGL(k_conv64<CMT>, dim3(B * H * (W / (64 * CMT))), dim3(256), 0, c->stream,
-                           c->dz[cur],
-                           c->wt + (size_t)(l - 1) * W64, (const float *)nullptr, c->z[l - 1],
-                           c->dz[1 - cur], (u16 *)nullptr, H, W, 1);
+        conv64(c, B, c->dz[cur], c->wt + (size_t)(l - 1) * W64, (const float *)nullptr,
+               c->z[l - 1], c->dz[1 - cur], (u16 *)nullptr, 1);
         CNN_TRY(launched(c, "k_conv64 (backward data)"));
         cur = 1 - cur;
     }
@@ -812,8 +876,8 @@ int s2d_cnn_forward_on(s2d_cnn *c, void *stream, int B, const float *w, const fl
     c->stream = static_cast<hipStream_t>(stream);
     int r = forward(c, B, w, psi, sN, ld);
     if (r == S2D_OK) {
-        hipLaunchKernelGGL(k_write_out, dim3(grid_for((long long)B * c->H * c->W, 256)),
-                           dim3(256), 0, c->stream, c->y, out, oN, old, c->out_scale, B, c->H,
+        hipLaunchKernelGGL(k_write_out, dim3(c->W / 128, B * c->H),
+                           dim3(128), 0, c->stream, c->y, out, oN, old, c->out_scale, B, c->H,
                            c->W, w, sN, ld, c->clip);
         r = launched(c, "k_write_out");
     }
@@ -828,7 +892,7 @@ int s2d_cnn_forward(s2d_cnn *c, int B, const float *w, const float *psi, long lo
     if (!w || !psi || !out) return cfail(c, S2D_ERR_INVALID_ARG, "null device pointer");
     CNN_HIP(c, hipSetDevice(c->device));
     CNN_TRY(forward(c, B, w, psi, sN, ld));
-    hipLaunchKernelGGL(k_write_out, dim3(grid_for((long long)B * c->H * c->W, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_write_out, dim3(c->W / 128, B * c->H), dim3(128), 0,
                        c->stream, c->y, out, oN, old, c->out_scale, B, c->H, c->W, w, sN, ld,
                        c->clip);
     CNN_TRY(launched(c, "k_write_out"));

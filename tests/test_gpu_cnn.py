@@ -38,24 +38,26 @@ def _setup(B=2, H=8, W=128, ld=None, seed=0):
     return net, p, x, t, xw, xp, tg, H * ld, ld
 
 
-@pytest.mark.parametrize("ld", [128, 130])
-def test_forward_matches_oracle(ld):
-    print(f"[cnn] forward vs oracle, ld={ld}", flush=True)
-    net, p, x, t, xw, xp, tg, stride, ld = _setup(ld=ld)
+# W = 128 and 384 take the 128-pixel conv tiles, W = 256 the 256-pixel ones
+@pytest.mark.parametrize("W,ld", [(128, 128), (128, 130), (256, 256), (384, 384)])
+def test_forward_matches_oracle(W, ld):
+    print(f"[cnn] forward vs oracle, W={W} ld={ld}", flush=True)
+    net, p, x, t, xw, xp, tg, stride, ld = _setup(W=W, ld=ld)
     out = torch.zeros_like(tg)
     net.forward(2, xw.data_ptr(), xp.data_ptr(), stride, ld, out.data_ptr(), stride, ld)
     torch.cuda.synchronize()
     y_ref, _ = ocnn.forward(p.astype(np.float64), x, in_scale=(0.8, 1.25))
-    got = out.cpu().numpy()[:, :, :128] / 0.5
+    got = out.cpu().numpy()[:, :, :W] / 0.5
     e = rel_l2(got, y_ref)
     print(f"  forward rel_l2 = {e:.2e}")
     assert e < 2e-3
     net.close()
 
 
-def test_gradient_matches_oracle():
-    print("[cnn] gradient vs oracle", flush=True)
-    net, p, x, t, xw, xp, tg, stride, ld = _setup(seed=3)
+@pytest.mark.parametrize("W", [128, 256])
+def test_gradient_matches_oracle(W):
+    print(f"[cnn] gradient vs oracle, W={W}", flush=True)
+    net, p, x, t, xw, xp, tg, stride, ld = _setup(W=W, seed=3)
     loss, g = net.gradient(2, xw.data_ptr(), xp.data_ptr(), stride, ld, tg.data_ptr(), stride, ld)
     loss_ref, g_ref, _ = ocnn.loss_and_grad(p.astype(np.float64), x, t, in_scale=(0.8, 1.25),
                                             out_scale=0.5)
