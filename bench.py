@@ -15,7 +15,7 @@ host cores for a bounded number of steps of the same workload.
 
 It also carries `closure`: the learned-closure speedup at the 5 % spectrum
 bound. Held-out members start 256^2 runs (the shipped closure network,
-refreshed every 3 steps) from the filtered 2048^2 DNS at t = 0.5; at t = 1
+refreshed every 8 steps) from the filtered 2048^2 DNS at t = 0.5; at t = 1
 the ensemble-mean enstrophy spectrum is compared with the filtered DNS, and
 the GPU time of the 2048^2 solve over [0.5, 1] is divided by the closure
 run's (both wall-clock on this GPU, one stream).
@@ -276,7 +276,7 @@ def run_closure(members: int):
         "gpu_s_dns": round(e["gpu_s_dns"], 3), "gpu_s_closure": round(e["gpu_s_closure"], 4),
         "steps": e["steps"],
         "config": {"workload": "held-out seeds 3000.., 2048^2 DNS (FP32, nu 2e-5, k0 12, CFL 0.5) "
-                               "vs 256^2 with the shipped CNN closure, refreshed every 3 steps, "
+                               "vs 256^2 with the shipped CNN closure, refreshed every 8 steps, "
                                "shell guard on; Z(k) = k^2 E(k) over shells 1..128"},
     }
 

@@ -124,53 +124,62 @@ class ClosedRun:
                                               float(nu), float(cfl), int(self.every)),
                     self.sim._h)
 
-    def advance_to(self, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 25,
+    def advance_to(self, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 100,
                    max_steps: int = 20000, z_limit: float = 4.0, log=None):
-        """Chunks of CFL steps until the time reaches t_end, then fixed steps
-        that land on it. Raises RuntimeError when the enstrophy exceeds
-        z_limit times its initial value or max_steps is reached (an unstable
-        closure)."""
+        """CFL steps in chunks (halved near the end) until the next CFL step
+        would reach t_end, then fixed steps that land on it. Raises
+        RuntimeError when the enstrophy exceeds z_limit times its initial
+        value or max_steps is reached (an unstable closure)."""
         s = self.sim
-        n = 0
-        t, _, dt = s.get_time()
-        if dt <= 0.0:
-            dt = s.ComputeTimeStep(cfl)
         z0 = s.diagnostics()["enstrophy"]
-        while t < t_end - 1e-12:
-            if n >= max_steps:
-                raise RuntimeError(f"no arrival at t = {t_end} within {max_steps} steps (t = {t})")
-            if t + chunk * 1.25 * dt < t_end:
-                self.steps(chunk, nu, cfl)
-                n += chunk
-                z = s.diagnostics()["enstrophy"]
-                if not (z <= z_limit * z0):
-                    raise RuntimeError(f"enstrophy grew {z / z0:.2f}x by t = {t:.4f}")
-                if log and n % (4 * chunk) == 0:
-                    log(f"  coarse run t = {t:.4f}, {n} steps, Z/Z0 = {z / z0:.3f}")
-            else:
-                if self.net is not None:
-                    self.closure_forcing()
-                s.rk3_steps(1, nu, min(s.ComputeTimeStep(cfl), t_end - t), 0.0)
-                n += 1
-            t, _, dt = s.get_time()
+
+        def cfl_steps(n):
+            self.steps(n, nu, cfl)
+            z = s.diagnostics()["enstrophy"]
+            if not (z <= z_limit * z0):
+                raise RuntimeError(f"enstrophy grew {z / z0:.2f}x by t = {s.get_time()[0]:.4f}")
+
+        def fixed_step(h):
+            if self.net is not None:
+                self.closure_forcing()
+            s.rk3_steps(1, nu, h, 0.0)
+
+        n = _advance(s, t_end, cfl, chunk, max_steps, cfl_steps, fixed_step)
+        if log:
+            log(f"  coarse run reached t = {s.get_time()[0]:.4f} in {n} steps")
         return n
 
 
-def advance_to(sim: Spectral2D, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 20):
-    """Fused CFL steps in chunks, then fixed steps that land on t_end."""
+def _advance(sim: Spectral2D, t_end: float, cfl: float, chunk: int, max_steps: int, cfl_steps,
+             fixed_step) -> int:
+    """Chunks of CFL steps while a chunk stays short of t_end (the chunk
+    halved as t_end nears, a step taken as at most 1.25 times the last
+    dt), then fixed steps of min(CFL dt, t_end - t) that land on t_end."""
     n = 0
     t, _, dt = sim.get_time()
     if dt <= 0.0:
         dt = sim.ComputeTimeStep(cfl)
+    c = chunk
     while t < t_end - 1e-12:
-        if t + chunk * 1.25 * dt < t_end:
-            sim.rk3_steps(chunk, nu, 0.0, cfl)
-            n += chunk
+        if n >= max_steps:
+            raise RuntimeError(f"no arrival at t = {t_end} within {max_steps} steps (t = {t})")
+        while c > 1 and t + c * 1.25 * dt >= t_end:
+            c //= 2
+        if t + c * 1.25 * dt < t_end:
+            cfl_steps(c)
+            n += c
         else:
-            sim.rk3_steps(1, nu, min(sim.ComputeTimeStep(cfl), t_end - t), 0.0)
+            fixed_step(min(sim.ComputeTimeStep(cfl), t_end - t))
             n += 1
         t, _, dt = sim.get_time()
     return n
+
+
+def advance_to(sim: Spectral2D, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 100):
+    """Fused CFL steps in chunks, then fixed steps that land on t_end."""
+    return _advance(sim, t_end, cfl, chunk, 1 << 30,
+                    lambda k: sim.rk3_steps(k, nu, 0.0, cfl),
+                    lambda h: sim.rk3_steps(1, nu, h, 0.0))
 
 
 def spectrum_error(z, z_ref, kmax: int):

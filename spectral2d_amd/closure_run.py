@@ -113,7 +113,7 @@ def load_net(path: str = SHIPPED, nc: int = 256):
 def eval_args(**kw):
     """The evaluation settings of main() (C4 defaults), overridable."""
     base = dict(nf=2048, nc=256, nu=2e-5, k0=12.0, cfl=0.5, eval_runs=64, eval_seed0=3000,
-                t0=0.5, t1=1.0, closure_every=3, guard=1)
+                t0=0.5, t1=1.0, closure_every=8, guard=1)
     base.update(kw)
     return argparse.Namespace(**base)
 
@@ -200,8 +200,9 @@ def main(argv=None):
                     help="1: keep only the enstrophy-sink part of tau (omega tau > 0)")
     ap.add_argument("--guard", type=int, default=1,
                     help="1: zero the closure forcing on backscattering shells")
-    ap.add_argument("--closure-every", type=int, default=3,
-                    help="closure evaluations every this many coarse steps")
+    ap.add_argument("--closure-every", type=int, default=8,
+                    help="closure evaluations every this many coarse steps (8: chosen on "
+                         "validation seeds 4000.. among 4, 8, 12)")
     ap.add_argument("--weights", help="skip data and training: evaluate these weights (e.g. "
                     "spectral2d_amd/data/closure_cnn_256.npz)")
     a = ap.parse_args(argv)
