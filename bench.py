@@ -57,7 +57,7 @@ def alg_bytes_per_launch(N: int, s: int):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
-PROFILE_EVERY = 10  # kernel timing: one step in ten of the timed region
+PROFILE_EVERY = 20  # kernel timing: one step in twenty of the timed region
 # kernel-name patterns of the three passes per workload
 KERNEL_TAGS = {"c1": ("k2_passA<", "k2_passB<", "k2_passC<"),
                "ensemble": ("k3_ens_passA<", "k3_ens_passB<", "k3_ens_passC<")}

@@ -61,8 +61,8 @@ struct WaveCfg {
     // leaves registers for more waves (measured: 10 % less time for the
     // mixed ensemble, 5 % at Np = 1536)
     static constexpr bool TW_PRE_A = sizeof(T) == 8;
-    // pass B: FP32 holds the twiddles of its five transforms in registers;
-    // FP64 has no room for them next to the data (256 VGPRs)
+    // pass B: FP32 holds the lane's twiddles in registers for all of its row
+    // transforms; FP64 has no room for them next to the data (256 VGPRs)
     static constexpr bool TW_PRE_B = sizeof(T) == 4;
     // pass B holds w and u + i v; a multi-wave group needs more registers
     static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
@@ -70,7 +70,8 @@ struct WaveCfg {
                   "unsupported group FFT size");
 };
 
-// Tiles of passes A and B: LPB lines of NP elements, one lane group each.
+// Tiles of passes A and B: LPB lines of NP elements (one lane group per
+// line, or per two rows in passB_pair).
 // The LDS slices are NP + 16 elements apart, so that the transposed reads
 // (LPB slices at one index) spread evenly over the banks.
 template <typename T, int NP, int LPB>
