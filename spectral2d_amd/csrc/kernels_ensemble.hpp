@@ -42,8 +42,10 @@ struct EnsArgs {
 template <typename T>
 struct EnsCfg {
     static constexpr int E = WaveCfg<T, 192>::E;
-    // lanes of a pass A / pass B block
-    static constexpr int BLK_A = 256, BLK_B = 256;
+    // lanes of a pass A / pass B block (pass A: 512, so that the 512^2
+    // class writes its G rows in 64-byte segments; measured 1.5 % less time
+    // per ensemble step than 256)
+    static constexpr int BLK_A = 512, BLK_B = 256;
     // lines per tile of passes A and B for padded size np
     static constexpr int lpb_a(int np) { return BLK_A * E / np; }
     static constexpr int lpb_b(int np) { return BLK_B * E / np; }
