@@ -332,7 +332,7 @@ def main(argv=None):
                          "object is printed alone); all: both on one line")
     ap.add_argument("--closure-members", type=int, default=4,
                     help="held-out members of the closure evaluation (0: skip)")
-    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--ens-warmup", type=int, default=10)
     args = ap.parse_args(argv)
 
