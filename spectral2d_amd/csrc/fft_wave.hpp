@@ -156,16 +156,21 @@ constexpr int early_tw_offset()
 
 // The per-lane twiddles of one lane-group transform: R - 1 per early pass
 // and E - 1 for the final pass, as forward values (an inverse transform
-// multiplies by their conjugates). With PRE they are loaded in one batch,
-// ahead of the transform's LDS exchanges (a load issued after an exchange's
-// fence cannot be moved above it, so loading per pass waits once per pass);
-// without PRE each pass loads its own, which holds fewer registers.
-template <typename T, int E, int TG, bool PRE = true>
+// multiplies by their conjugates). PRE selects which are loaded in one
+// batch ahead of the transform's LDS exchanges (bit 0: the early passes',
+// bit 1: the final pass's); the others are loaded by their pass. A load
+// issued after an exchange's fence cannot be moved above it, so per-pass
+// loads wait once per pass; batched ones hold registers for the whole
+// transform.
+constexpr int TW_EARLY = 1, TW_FINAL = 2, TW_ALL = 3;
+
+template <typename T, int E, int TG, int PRE = TW_ALL>
 struct GroupTw {
     static constexpr int n = E * TG;
     static constexpr int NE = early_tw_offset<E, TG, TG>();
-    cpx<T> e[PRE && NE > 0 ? NE : 1];
-    cpx<T> f[PRE ? E : 1];
+    static constexpr bool PE = (PRE & TW_EARLY) != 0, PF = (PRE & TW_FINAL) != 0;
+    cpx<T> e[PE && NE > 0 ? NE : 1];
+    cpx<T> f[PF ? E : 1];
     const cpx<T> *tw;
     int t;
 
@@ -173,8 +178,8 @@ struct GroupTw {
     {
         tw = tw_;
         t = t_;
-        if constexpr (PRE) {
-            load_early<1>();
+        if constexpr (PE) load_early<1>();
+        if constexpr (PF) {
 #pragma unroll
             for (int m = 1; m < E; ++m) f[m] = tw[m * t];
         }
@@ -183,13 +188,13 @@ struct GroupTw {
     template <int Ls, int R>
     __device__ __forceinline__ cpx<T> early(int m) const
     {
-        if constexpr (PRE) return e[early_tw_offset<E, TG, Ls>() + m - 1];
+        if constexpr (PE) return e[early_tw_offset<E, TG, Ls>() + m - 1];
         else return tw[m * (t & (Ls - 1)) * (n / (Ls * R))];
     }
     // twiddle m (1 <= m < E) of the final pass
     __device__ __forceinline__ cpx<T> final(int m) const
     {
-        if constexpr (PRE) return f[m];
+        if constexpr (PF) return f[m];
         else return tw[m * t];
     }
 
@@ -271,7 +276,7 @@ __device__ __forceinline__ void final_pass(cpx<T> (&v)[E], const G &g)
 // Full transform of the group: on entry v[s] = x[t + TG*s], on exit
 // v[s] = X[t + TG*s]. lds is the group's slice (LdsPad<T>::size(n)); g holds
 // the lane's twiddles (GroupTw::load with the table of length n).
-template <typename T, int E, int TG, int D, bool PRE>
+template <typename T, int E, int TG, int D, int PRE>
 __device__ __forceinline__ void group_fft(cpx<T> (&v)[E], cpx<T> *lds, int t,
                                           const GroupTw<T, E, TG, PRE> &g)
 {

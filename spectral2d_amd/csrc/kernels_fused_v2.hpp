@@ -55,15 +55,15 @@ struct WaveCfg {
     static constexpr int WPE_C = sizeof(T) == 8 ? 2 : 4;
     // pass C: twiddles in one batch, and the compact-state operands loaded
     // ahead of the transform
-    static constexpr bool TW_PRE_C = true;
+    static constexpr int TW_PRE_C = TW_ALL;
     static constexpr bool OPS_EARLY_C = true;
     // pass A: FP64 loads its twiddles in one batch; FP32 per pass, which
     // leaves registers for more waves (measured: 10 % less time for the
     // mixed ensemble, 5 % at Np = 1536)
-    static constexpr bool TW_PRE_A = sizeof(T) == 8;
+    static constexpr int TW_PRE_A = sizeof(T) == 8 ? TW_ALL : 0;
     // pass B: FP32 holds the lane's twiddles in registers for all of its row
     // transforms; FP64 has no room for them next to the data (256 VGPRs)
-    static constexpr bool TW_PRE_B = sizeof(T) == 4;
+    static constexpr int TW_PRE_B = sizeof(T) == 4 ? TW_ALL : 0;
     // pass B holds w and u + i v; a multi-wave group needs more registers
     static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
