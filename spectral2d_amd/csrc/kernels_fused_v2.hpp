@@ -48,19 +48,22 @@ struct WaveCfg {
     static constexpr int GPB = TG >= 64 ? 1 : 64 / TG;
     static constexpr int BLK = TG >= 64 ? TG : 64; // threads per half
     static constexpr int SLICE = LdsPad<T>::size(NP);
-    // register budgets, waves per SIMD, of passes A and C (FP64: 256 VGPRs
-    // hold a transform's data, its twiddles and the loads issued ahead of
-    // it without spills)
-    static constexpr int WPE_A = sizeof(T) == 8 ? 2 : 4;
+    // register budgets, waves per SIMD, of passes A and C (FP64 pass C: 256
+    // VGPRs hold a transform's data, its twiddles and the loads issued ahead
+    // of it without spills; FP64 pass A fits three waves in 168 by loading
+    // only the early-pass twiddles in one batch)
+    static constexpr int WPE_A = sizeof(T) == 8 ? 3 : 4;
     static constexpr int WPE_C = sizeof(T) == 8 ? 2 : 4;
     // pass C: twiddles in one batch, and the compact-state operands loaded
     // ahead of the transform
     static constexpr int TW_PRE_C = TW_ALL;
     static constexpr bool OPS_EARLY_C = true;
-    // pass A: FP64 loads its twiddles in one batch; FP32 per pass, which
-    // leaves registers for more waves (measured: 10 % less time for the
-    // mixed ensemble, 5 % at Np = 1536)
-    static constexpr int TW_PRE_A = sizeof(T) == 8 ? TW_ALL : 0;
+    // pass A: FP64 loads its early-pass twiddles in one batch and the final
+    // pass's per use (1.3 % less time at Np = 1536 than all in one batch at
+    // two waves); FP32 loads all per pass, which leaves registers for more
+    // waves (measured: 10 % less time for the mixed ensemble, 5 % at
+    // Np = 1536)
+    static constexpr int TW_PRE_A = sizeof(T) == 8 ? TW_EARLY : 0;
     // pass B: FP32 holds the lane's twiddles in registers for all of its row
     // transforms; FP64 has no room for them next to the data (256 VGPRs)
     static constexpr int TW_PRE_B = sizeof(T) == 4 ? TW_ALL : 0;
