@@ -6,13 +6,16 @@
 !! argument) instead of compile-time PARAMETERs:
 !!
 !!   &spectral2d nx=256, ny=256, nu=0.01, dt=1.0e-3, cfl=0.0, nsteps=1000,
-!!     ic='taylor-green', seed=1, k0=10.0, ke=0.5, printPeriod=100,
+!!     tEnd=0.0, ic='taylor-green', seed=1, k0=10.0, ke=0.5, printPeriod=100,
 !!     deviceResident=.TRUE., out='' /
 !!
 !! ic is 'taylor-green' (SetTaylorGreen) or 'turbulence' (the seeded
 !! replacement of SetTurbulentVelocity). cfl > 0 recomputes dt on the GPU
-!! every step. out names a raw file receiving the final wC (cISize x nyP
-!! complex doubles, Fortran order) for checking.
+!! every step. tEnd > 0 ends the run at that time instead of after nsteps
+!! steps (the reference's DESIRED_END_TIME, src/Spectral2D.F90:426-430: once
+!! time + dt reaches tEnd, one last step of tEnd - time). out names a raw
+!! file receiving the final wC (cISize x nyP complex doubles, Fortran order)
+!! for checking.
 PROGRAM Spectral2D_p
 
    USE ISO_C_BINDING
@@ -32,11 +35,12 @@ PROGRAM Spectral2D_p
    INTEGER(KIND=IWPF) :: nx = 96, ny = 96, nsteps = 100, printPeriod = 10
    INTEGER(KIND=IWPF) :: seed = 1
    REAL(KIND=RWPC) :: nu = 0.005_RWPC, dt = 1.0e-5_RWPC, cfl = 0.0_RWPC
+   REAL(KIND=RWPF) :: tEnd = 0.0_RWPF
    REAL(KIND=RWPF) :: k0 = 10.0_RWPF, ke = 0.5_RWPF
    CHARACTER(LEN=32) :: ic = 'taylor-green'
    CHARACTER(LEN=256) :: out = ''
    LOGICAL :: deviceResident = .TRUE.
-   NAMELIST /spectral2d/ nx, ny, nu, dt, cfl, nsteps, ic, seed, k0, ke, &
+   NAMELIST /spectral2d/ nx, ny, nu, dt, cfl, nsteps, tEnd, ic, seed, k0, ke, &
                          printPeriod, deviceResident, out
 
    INTEGER(KIND=IWPF) :: nxG, nyG, kxLG, kyLG, kxMG, kyMG, kxLU, kyLU, kxMU, kyMU
@@ -47,6 +51,7 @@ PROGRAM Spectral2D_p
    REAL(KIND=RWPC),DIMENSION(:,:),ALLOCATABLE :: wR, psiR, uR, vR
    REAL(KIND=RWPF) :: time = 0.0_RWPF
    INTEGER(KIND=IWPF) :: nadv = 0, i, j, u, ios
+   LOGICAL :: exitThisStep = .FALSE.
    CHARACTER(LEN=512) :: cfg
    TYPE(s2d_diag_t) :: diag
 
@@ -109,8 +114,10 @@ PROGRAM Spectral2D_p
    CALL SetDeviceResident(deviceResident)
    cflNumber = cfl
 
-   ! time loop, src/Spectral2D.F90:338-432 (fixed step count; print cadence)
-   DO WHILE (nadv < nsteps)
+   ! time loop, src/Spectral2D.F90:338-432: print cadence, then the exit
+   ! conditions (FIXED_STEP_COUNT with tEnd <= 0, else DESIRED_END_TIME)
+   tloop: DO
+      IF (tEnd <= 0.0_RWPF .AND. nadv >= nsteps) EXIT tloop
       IF (cfl > 0.0_RWPC) THEN
          CALL IntegrateOneStepCfl()
       ELSE
@@ -119,7 +126,8 @@ PROGRAM Spectral2D_p
          time = time + dt
          nadv = nadv + 1_IWPF
       END IF
-      IF (MOD(nadv, printPeriod) == 0 .OR. nadv == nsteps) THEN
+      IF (MOD(nadv, printPeriod) == 0 .OR. (tEnd <= 0.0_RWPF .AND. nadv == nsteps) &
+          .OR. exitThisStep) THEN
          CALL SyncHostFields(wC, psiC)
          ! spectral magnitudes, as the reference prints (src/Spectral2D.F90:358-361)
          WRITE(OUTPUT_UNIT,'(A,I8.8,A,ES15.8,A,ES15.8,A,ES15.8,A,ES15.8,A,ES15.8)') &
@@ -127,7 +135,16 @@ PROGRAM Spectral2D_p
             '; Max W: ', MAXVAL(ABS(wC)), '; Min W: ', MINVAL(ABS(wC)), &
             '; Max Psi: ', MAXVAL(ABS(psiC)), '; Min Psi: ', MINVAL(ABS(psiC))
       END IF
-   END DO
+      IF (tEnd > 0.0_RWPF) THEN
+         IF (exitThisStep) EXIT tloop
+         ! fixed dt: the next step is the last once it would reach tEnd
+         ! (with a CFL dt the clamp follows ComputeTimeStep instead)
+         IF (cfl <= 0.0_RWPC .AND. time + dt >= tEnd) THEN
+            exitThisStep = .TRUE.
+            dt = tEnd - time
+         END IF
+      END IF
+   END DO tloop
 
    CALL S2D_Check(s2d_diagnostics(s2dCtx, diag), s2dCtx, 'diagnostics')
    WRITE(OUTPUT_UNIT,'(A,ES24.16,A,ES24.16,A,ES24.16)') 'Final TKE: ', diag%tke, &
@@ -150,6 +167,10 @@ CONTAINS
    !! device (ComputeTimeStep fills the reference's empty stub).
    SUBROUTINE IntegrateOneStepCfl()
       CALL ComputeTimeStep(dt)
+      IF (tEnd > 0.0_RWPF .AND. time + dt >= tEnd) THEN
+         exitThisStep = .TRUE.
+         dt = tEnd - time
+      END IF
       CALL IntegrateOneStep(nxG, nyG, nxP, nyP, rISize, cISize, kxG, kyG, &
                             nu, dt, uC, uR, vC, vR, wC, wR, psiC, psiR)
       time = time + dt

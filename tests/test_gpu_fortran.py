@@ -54,6 +54,60 @@ def test_fortran_taylor_green(tmp_path):
     assert "Simulation step number: 00000200" in out
 
 
+def test_fortran_end_time_fixed_dt(tmp_path):
+    # DESIRED_END_TIME (src/Spectral2D.F90:426-430): after the step that
+    # leaves time + dt >= tEnd, one last step of tEnd - time
+    N, nu, dt, t_end = 64, 0.01, 3e-3, 0.01
+    w, out = run_driver(tmp_path, nx=N, ny=N, nu=nu, dt=dt, tEnd=t_end, nsteps=1,
+                        ic="taylor-green", printPeriod=100)
+    o = oracle.Oracle(N)
+    st = o.new_state()
+    o.set_taylor_green(st["w"], st["psi"])
+    w0 = o.cview(st["w"]).copy()
+    t, steps, d = 0.0, 0, dt
+    while True:
+        o.step(st, nu, d)
+        t += d
+        steps += 1
+        if t + d >= t_end:
+            d = t_end - t
+            o.step(st, nu, d)
+            t += d
+            steps += 1
+            break
+    assert steps == 4 and abs(t - t_end) < 1e-15
+    got = w.reshape(o.nyG, o.cISize)
+    assert rel_l2(got, o.cview(st["w"])) < 1e-12
+    assert rel_l2(got, w0 * np.exp(-2 * nu * t_end)) < 1e-12
+    assert "Simulation step number: 00000004" in out
+    final = [l for l in out.splitlines() if l.startswith("Final TKE")][0]
+    assert abs(float(final.split("time:")[1]) - t_end) < 1e-15
+
+
+def test_fortran_end_time_cfl(tmp_path):
+    # with a CFL dt the clamp follows ComputeTimeStep: min(CFL dt, tEnd - time)
+    N, nu, t_end = 64, 1e-3, 0.05
+    w, out = run_driver(tmp_path, nx=N, ny=N, nu=nu, cfl=0.5, tEnd=t_end, nsteps=1,
+                        ic="turbulence", seed=5, k0=8.0, ke=0.5, printPeriod=1000)
+    o = oracle.Oracle(N)
+    st = o.new_state()
+    o.set_turbulence(st["w"], st["psi"], seed=5, k0=8.0, ke=0.5)
+    t, steps = 0.0, 0
+    while True:
+        d = o.cfl_dt(st["psi"], 0.5)
+        last = t + d >= t_end
+        if last:
+            d = t_end - t
+        o.step(st, nu, d)
+        t += d
+        steps += 1
+        if last:
+            break
+    assert steps > 3
+    assert rel_l2(w.reshape(o.nyG, o.cISize), o.cview(st["w"])) < 1e-12
+    assert f"Simulation step number: {steps:08d}" in out
+
+
 @pytest.mark.parametrize("resident", [True, False])
 def test_fortran_turbulence_cfl(tmp_path, resident):
     N, nu, n = 64, 1e-3, 30
