@@ -47,11 +47,35 @@ __device__ __forceinline__ void group_sync()
 // used). It keeps the strided Stockham writes and the contiguous reads
 // within 1.25-1.5x of conflict-free for both 32- and 64-bank LDS, with no
 // padding.
+//
+// Every exchange addresses slot b + c of a lane-dependent b and a constant c
+// that share no set bit. idx distributes over such sums as idx(b) ^ idx(c),
+// and as it only changes bits 0-3 that is (idx(b) ^ (idx(c) & 15)) +
+// (c & ~15): with idx(b) computed once per pass, an access costs one XOR
+// with a constant, and the addition folds into the LDS instruction's offset.
 template <typename T>
 struct LdsPad {
-    static __device__ __forceinline__ int idx(int i) { return i ^ ((i >> 3) & 15); }
+    static __host__ __device__ constexpr int idx(int i) { return i ^ ((i >> 3) & 15); }
+    // idx(b + c) from ib = idx(b), for b & c == 0
+    static __host__ __device__ constexpr int idx_sum(int ib, int c)
+    {
+        return (ib ^ (idx(c) & 15)) + (c & ~15);
+    }
     static constexpr int size(int n) { return n; }
 };
+
+// idx_sum against idx over every pair of disjoint 8-bit values (idx reads
+// bits 0-6 only, so higher bits just add).
+constexpr bool lds_idx_sum_exact()
+{
+    for (int b = 0; b < 256; ++b)
+        for (int c = 0; c < 256; ++c)
+            if ((b & c) == 0 && LdsPad<float>::idx_sum(LdsPad<float>::idx(b), c) !=
+                                    LdsPad<float>::idx(b + c))
+                return false;
+    return true;
+}
+static_assert(lds_idx_sum_exact(), "idx_sum differs from idx");
 
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [B, N).
 template <int B, int N, typename F>
@@ -230,10 +254,11 @@ __device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, c
 {
     constexpr int Q = E / R;
     const int k = t & (Ls - 1);
+    // slot of output m of butterfly q: (t - k) R + k, the lane's part, plus
+    // TG q R + m Ls, which shares no bit with it
+    const int ib = LdsPad<T>::idx((t - k) * R + k);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int j = t + TG * q;
-        const int base = (j - k) * R + k;
         cpx<T> a[R];
 #pragma unroll
         for (int m = 0; m < R; ++m) a[m] = v[q + m * Q];
@@ -243,7 +268,7 @@ __device__ __forceinline__ void group_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, c
         }
         dft<T, R, D>(a);
 #pragma unroll
-        for (int m = 0; m < R; ++m) lds[LdsPad<T>::idx(base + m * Ls)] = a[m];
+        for (int m = 0; m < R; ++m) lds[LdsPad<T>::idx_sum(ib, TG * q * R + m * Ls)] = a[m];
     }
 }
 
@@ -255,8 +280,9 @@ __device__ __forceinline__ void early_passes(cpx<T> (&v)[E], cpx<T> *lds, int t,
         static_assert(E % R == 0 && (TG / Ls) % R == 0, "no early radix");
         group_pass<T, E, TG, R, Ls, D, G>(v, lds, t, g);
         group_sync<TG>();
+        const int it = LdsPad<T>::idx(t);
 #pragma unroll
-        for (int s = 0; s < E; ++s) v[s] = lds[LdsPad<T>::idx(t + TG * s)];
+        for (int s = 0; s < E; ++s) v[s] = lds[LdsPad<T>::idx_sum(it, TG * s)];
         group_sync<TG>();
         early_passes<T, E, TG, D, Ls * R, G>(v, lds, t, g);
     }

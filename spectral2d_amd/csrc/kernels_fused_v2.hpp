@@ -273,8 +273,9 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     cpx<T> *slice = lds + g * SL;
     group_fft<T, E, TG, 1>(v, slice, t, gt);
     // transpose through LDS: the block's columns become LPB-element row segments
+    const int it = LdsPad<T>::idx(t);
 #pragma unroll
-    for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx(t + TG * s)] = v[s];
+    for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx_sum(it, TG * s)] = v[s];
     __syncthreads();
     // LPB * NP = E * BLK elements: element e = threadIdx.x + BLK m is row
     // e / LPB, column e % LPB of the tile (BLK = LPB * TG)
@@ -282,6 +283,8 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     if (kx0 + gi < d.K) {
         cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx0 + gi;
         const cpx<T> *sl = lds + gi * SL;
+        // (the full index here: with idx_sum the mixed ensemble's pass A took
+        // 12 % longer)
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = yb + TG * m;
@@ -415,8 +418,9 @@ __device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
     for (int s = 0; s < E; ++s) v[s] = mk<T>(v[s].re * wr[s], v[s].im * wr[s]); // uw + i vw
     if (forward) {
         group_fft<T, E, TG, -1>(v, slice, t, gt);
+        const int it = LdsPad<T>::idx(t);
 #pragma unroll
-        for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx(t + TG * s)] = v[s];
+        for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx_sum(it, TG * s)] = v[s];
         __syncthreads();
         // k <= N/2 = NP/3 of the block's LPB rows, rows fastest: LPB-element
         // segments of the H columns
@@ -525,8 +529,9 @@ __device__ __forceinline__ void passB_pair(int blk, int nblk, cpx<T> *lds,
 #pragma unroll
             for (int s = 0; s < E; ++s) x[s] = mk<T>(x[s].re * w[s], x[s].im * w[s]);
             group_fft<T, E, TG, -1>(x, sl, t, gt);
+            const int it = LdsPad<T>::idx(t);
 #pragma unroll
-            for (int s = 0; s < E; ++s) sl[LdsPad<T>::idx(t + TG * s)] = x[s];
+            for (int s = 0; s < E; ++s) sl[LdsPad<T>::idx_sum(it, TG * s)] = x[s];
         }
     };
     if constexpr (sizeof(T) != 4) load_uv(z, oa);
@@ -646,6 +651,7 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
     const T qx = (T)kx;
     const T inv_norm = (T)a.inv_norm; // multiplications instead of the
                                       // reference's divisions: <= 1 ulp
+    const int it = LdsPad<T>::idx(t);
     static_for<0, E / 2>([&](auto iI) {
         constexpr int i = decltype(iI)::value, s = 2 * i + HALF;
         if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
@@ -655,7 +661,7 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
             // SetupTruncation's |k| <= kMax + 1e-10 (src/Spectral.F90:272-313)
             // is kx^2 + ky^2 <= (N/2)^2 on integer wavenumbers
             const bool keep = kx * kx + kyi * kyi <= H * H;
-            const cpx<T> o = other[LdsPad<T>::idx(j)];
+            const cpx<T> o = other[LdsPad<T>::idx_sum(it, TG * s)]; // j = t + TG s
             const cpx<T> Pr = HALF == 0 ? v[s] : o, Qr = HALF == 0 ? o : v[s];
             const cpx<T> P = mk<T>(Pr.re * inv_norm, Pr.im * inv_norm);
             const cpx<T> Q = mk<T>(Qr.re * inv_norm, Qr.im * inv_norm);
@@ -717,9 +723,10 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     group_fft<T, E, TG, -1>(v, mine, t, gt);
     if constexpr (!C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
     // half 0 keeps the even s and hands over the odd s, half 1 the reverse
+    const int it = LdsPad<T>::idx(t);
 #pragma unroll
     for (int s = 0; s < E; ++s)
-        if ((s & 1) != HALF) mine[LdsPad<T>::idx(t + TG * s)] = v[s];
+        if ((s & 1) != HALF) mine[LdsPad<T>::idx_sum(it, TG * s)] = v[s];
     __syncthreads();
     passC_finish<T, NP, HALF>(v, other, t, kx, active, op, Fc != nullptr, Wout, col, a, dtd, nu);
     if (HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0) {
