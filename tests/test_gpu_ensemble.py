@@ -169,3 +169,31 @@ def test_c2_shape_smoke():
         assert d1[r]["tke"] < d0[r]["tke"]
     assert bytes_per_step([512]) == 60_002_304  # the two-pass model, 512^2 FP32
     e.close()
+
+
+def test_c2_full_draw_members_equal_runs_alone():
+    """The measured C2 workload itself (bench.ensemble_draw: 256 runs of
+    N in {128, 256, 512}), 20 graph-replayed CFL steps: every run advances,
+    stays finite and loses energy, and the first run of each size equals the
+    same run computed alone, bitwise."""
+    from bench import ensemble_draw
+    sizes, k0, nu, seeds = ensemble_draw()
+    steps = 20
+    print(f"[ensemble] C2 draw, {len(sizes)} runs, counts "
+          f"{[int((sizes == n).sum()) for n in (128, 256, 512)]}, {steps} graph steps", flush=True)
+    e = _ensemble(sizes, FP32, seeds, k0, nu)
+    d0 = e.diagnostics()
+    e.steps(steps, cfl=0.5, graph=True)
+    d1 = e.diagnostics()
+    for r in range(len(sizes)):
+        assert d1[r]["nadv"] == steps and d1[r]["time"] > 0, r
+        assert np.isfinite(d1[r]["tke"]) and d1[r]["tke"] < d0[r]["tke"], r
+    for N in (128, 256, 512):
+        r = int(np.flatnonzero(sizes == N)[0])
+        s = Spectral2D(int(N), precision=FP32)
+        s.SetTurbulence(int(seeds[r]), float(k0[r]), 0.5)
+        s.rk3_steps(steps, float(nu[r]), 0.0, 0.5)
+        assert np.array_equal(e.get_field(r, "w"), s.get_field("w")), (r, N)
+        assert d1[r]["time"] == s.diagnostics()["time"], (r, N)
+        s.close()
+    e.close()
