@@ -198,6 +198,23 @@ struct Batch {
 // ------------------------------------------------------------- pass A
 // Jobs: (run, field, tile of LPB adjacent kx) or (run, tile, field); tiles
 // past K hold inactive columns (zero input, nothing stored).
+//
+// The last column kx = K - 1 = N/2 holds a single kept mode, ky = 0 (the
+// compact state is truncated to the disk |k| <= N/2), so its three fields
+// are constant along y (the inverse transform of a lone ky = 0 mode is
+// that mode). When LPB divides K - 1 the blocks of the last full tile write
+// it as such, instead of a tile of its own that would transform one active
+// column.
+__host__ __device__ constexpr bool passA_tail(int K, int LPB)
+{
+    return K > 1 && (K - 1) % LPB == 0;
+}
+
+__host__ __device__ constexpr int passA_tiles(int K, int LPB)
+{
+    return passA_tail(K, LPB) ? (K - 1) / LPB : (K + LPB - 1) / LPB;
+}
+
 template <typename T, int NP, int LPB>
 __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Wsrc,
@@ -211,7 +228,7 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     using Tl = TileCfg<T, NP, LPB>;
     constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL;
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
-    const int tiles = (d.K + LPB - 1) / LPB;
+    const int tiles = passA_tiles(d.K, LPB);
     // XCD-major blocks: the jobs in flight on one XCD are adjacent
     const int job = xcd_major(blk, nblk);
     if (job >= 3 * tiles * bt.nruns) return; // the whole block
@@ -291,13 +308,26 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
             col[(size_t)y * d.KG] = sl[LdsPad<T>::idx(y)];
         }
     }
+    if (passA_tail(d.K, LPB) && kx0 + LPB == d.K - 1) { // block-uniform
+        // column K - 1 of field f: its ky = 0 mode (compact row 0) taken
+        // through the same spectrum arithmetic as a transformed column
+        const int kt = d.K - 1;
+        const cpx<T> w0 = Wsrc[run * bt.sW + (size_t)kt * d.KP];
+        const T qx = (T)kt;
+        const T inv = recip(qx * qx);
+        const cpx<T> ps = mk<T>(w0.re * inv, w0.im * inv);
+        const T a = (f == 0) ? (T)0 : -qx;
+        const cpx<T> val = f == 2 ? w0 : mk<T>(-(a * ps.im), a * ps.re);
+        cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kt;
+        for (int y = threadIdx.x; y < NP; y += BLK) col[(size_t)y * d.KG] = val;
+    }
 }
 
 // Blocks of pass A for nruns runs.
 template <typename T, int NP, int LPB>
 __host__ __device__ constexpr int passA_blocks(int K, int nruns)
 {
-    return 3 * ((K + LPB - 1) / LPB) * nruns;
+    return 3 * passA_tiles(K, LPB) * nruns;
 }
 
 template <typename T, int NP>
