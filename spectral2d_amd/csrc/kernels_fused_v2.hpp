@@ -664,6 +664,26 @@ struct PassCOperands {
     }
 };
 
+// The Shu-Osher stage update of one mode (src/TimeIntegration.F90:156-202)
+// from the step-start value w0, the stage value ws and dW.
+template <typename T>
+__device__ __forceinline__ cpx<T> rk3_update(int stage, T dt, cpx<T> w0v, cpx<T> wsv, cpx<T> dW)
+{
+#pragma clang fp contract(off)
+    if (stage == 1)
+        return mk<T>(w0v.re + dt * dW.re, w0v.im + dt * dW.im);
+    if (stage == 2) {
+        const T cc = (T)0.25 * dt;
+        return mk<T>(((T)0.75 * w0v.re + (T)0.25 * wsv.re) + cc * dW.re,
+                     ((T)0.75 * w0v.im + (T)0.25 * wsv.im) + cc * dW.im);
+    }
+    const T cc = dt * (T)2;
+    // (w + 2 w2 + 2 dt dW)/3 correctly rounded: a rounded 1/3 multiplier
+    // alone would bias |w| by its rounding error every step
+    return mk<T>(div3(((w0v.re + (T)2 * wsv.re) + cc * dW.re)),
+                 div3(((w0v.im + (T)2 * wsv.im) + cc * dW.im)));
+}
+
 // The RHS, truncation and stage update of half HALF's rows of column kx;
 // P / Q are the uw / vw column spectra (one of them in registers, the other
 // read from the partner half's LDS slice).
@@ -701,29 +721,27 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E
             const cpx<T> wsv = op.ws[i], w0v = op.w0[i];
             cpx<T> dW = mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
             if (forced) dW = mk<T>(dW.re - op.fv[i].re, dW.im - op.fv[i].im); // closure / forcing
-            cpx<T> out;
-            if (a.stage == 1) {
-                out = mk<T>(w0v.re + dt * dW.re, w0v.im + dt * dW.im);
-            } else if (a.stage == 2) {
-                const T cc = (T)0.25 * dt;
-                out = mk<T>(((T)0.75 * w0v.re + (T)0.25 * wsv.re) + cc * dW.re,
-                            ((T)0.75 * w0v.im + (T)0.25 * wsv.im) + cc * dW.im);
-            } else {
-                const T cc = dt * (T)2;
-                // (w + 2 w2 + 2 dt dW)/3 correctly rounded: a rounded 1/3
-                // multiplier alone would bias |w| by its rounding error
-                // every step
-                out = mk<T>(div3(((w0v.re + (T)2 * wsv.re) + cc * dW.re)),
-                            div3(((w0v.im + (T)2 * wsv.im) + cc * dW.im)));
-            }
+            cpx<T> out = rk3_update<T>(a.stage, dt, w0v, wsv, dW);
             out = sel(keep, out, mk<T>((T)0, (T)0));
             if (active && op.row[i]) Wout[col + op.c[i]] = out;
         }
     });
 }
 
+// Column K - 1 written by passC_tail (one sum instead of a transformed
+// column) for the sizes above the ensemble's (Np >= 1536): at Np = 1536 in
+// FP64 two blocks fit a CU, and the 513th column ran as a second round
+// (measured: pass C 13 % less time in FP64, 6 % in FP32); in the
+// ensemble, whose sizes keep the plain columns, the per-run tails made
+// pass C 8 % slower.
+template <int NP>
+constexpr bool passC_tail_on()
+{
+    return NP >= 1536;
+}
+
 // One half of a pass C block: half 0 transforms uw, half 1 vw.
-template <typename T, int NP, int HALF>
+template <typename T, int NP, int HALF, bool TAIL>
 __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
                                            const cpx<T> *__restrict__ Hsrc, const cpx<T> *Wn,
                                            const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
@@ -736,8 +754,9 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     cpx<T> *mine = lds_all + (HALF * C::GPB + g) * C::SLICE;
     const cpx<T> *other = lds_all + ((1 - HALF) * C::GPB + g) * C::SLICE;
     const int job = blk * C::GPB + g;
-    const bool active = job < d.K * bt.nruns;
-    const int run = active ? job / d.K : 0, kx = active ? job - run * d.K : 0;
+    const int Kt = TAIL ? d.K - 1 : d.K; // transformed columns
+    const bool active = job < Kt * bt.nruns;
+    const int run = active ? job / Kt : 0, kx = active ? job - run * Kt : 0;
     const cpx<T> *src = Hsrc + run * bt.sH + (size_t)kx * NP;
     cpx<T> v[E];
 #pragma unroll
@@ -766,7 +785,51 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     }
 }
 
+// Column kx = K - 1 = N/2 of run `run`: its one kept mode is ky = 0 (the
+// compact state is truncated to the disk |k| <= N/2), whose transform
+// term is the plain sum over y of the H_uw column (the vw term enters as
+// i ky = 0). The sum runs in a fixed order (strided partial sums, then the
+// block's reduction), the RHS and stage update as passC_finish's, and the
+// column's other rows are written as zeros. Every thread of the block
+// takes part.
 template <typename T, int NP>
+__device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T> *Huw,
+                                           const cpx<T> *Wn, const cpx<T> *Wst, cpx<T> *Wout,
+                                           const FusedDims &d, DevScalars *sc,
+                                           const StageArgs &a, const Batch &bt)
+{
+#pragma clang fp contract(off)
+    constexpr int NT = 2 * WaveCfg<T, NP>::BLK, NW = NT / 64;
+    const int kx = d.K - 1;
+    const cpx<T> *h = Huw + run * bt.sH + (size_t)kx * NP;
+    cpx<T> p = mk<T>((T)0, (T)0);
+    for (int y = threadIdx.x; y < NP; y += NT) p = add(p, h[y]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        p = mk<T>(p.re + __shfl_xor(p.re, off), p.im + __shfl_xor(p.im, off));
+    __syncthreads(); // the slices' last reads (passC_finish) are done
+    if ((threadIdx.x & 63) == 0) lds_all[threadIdx.x / 64] = p;
+    __syncthreads();
+    const size_t col = run * bt.sW + (size_t)kx * d.KP;
+    if (threadIdx.x == 0) {
+        cpx<T> P = lds_all[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) P = add(P, lds_all[w]);
+        const T inv_norm = (T)a.inv_norm;
+        P = mk<T>(P.re * inv_norm, P.im * inv_norm);
+        const T qx = (T)kx;
+        const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
+        const T sn = (T)(bt.nu ? bt.nu[run] : a.nu) * (qx * qx);
+        const cpx<T> wsv = Wst[col], w0v = Wn[col];
+        cpx<T> dW = mk<T>((-du.re) - sn * wsv.re, (-du.im) - sn * wsv.im);
+        const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
+        if (Fc) dW = mk<T>(dW.re - Fc[col].re, dW.im - Fc[col].im);
+        Wout[col] = rk3_update<T>(a.stage, (T)stage_dt(a, sc + run), w0v, wsv, dW);
+    }
+    for (int c = 1 + threadIdx.x; c <= d.N; c += NT) Wout[col + c] = mk<T>((T)0, (T)0);
+}
+
+template <typename T, int NP, bool TAIL = false>
 __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
                                            const cpx<T> *__restrict__ Huw,
                                            const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
@@ -778,9 +841,16 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
     const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK;
     // wave-uniform; both halves pass the same barriers
     if (half == 0)
-        passC_half<T, NP, 0>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);
+        passC_half<T, NP, 0, TAIL>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);
     else
-        passC_half<T, NP, 1>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
+        passC_half<T, NP, 1, TAIL>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
+    if constexpr (TAIL) {
+        // the block holding column 0 of a run also writes the run's column K - 1
+        const int Kt = d.K - 1, j0 = blk * C::GPB;
+        const int tr = (j0 + Kt - 1) / Kt; // first run whose column 0 is at or after j0
+        if (tr < bt.nruns && tr * Kt < j0 + C::GPB) // block-uniform
+            passC_tail<T, NP>(tr, lds_all, Huw, Wn, Wst, Wout, d, sc, a, bt);
+    }
 }
 
 template <typename T, int NP>
@@ -792,7 +862,8 @@ k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const c
 {
     using C = WaveCfg<T, NP>;
     __shared__ cpx<T> lds_all[2 * C::GPB * C::SLICE];
-    passC_body<T, NP>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
+    passC_body<T, NP, passC_tail_on<NP>()>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw,
+                                          sc, a, bt);
 }
 
 } // namespace s2d

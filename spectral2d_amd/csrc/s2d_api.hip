@@ -670,7 +670,10 @@ int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &
 {
     using Cf = WaveCfg<T, NP>;
     FusedDims d = fdims(c);
-    const int blocks = (d.K * c->nruns + Cf::GPB - 1) / Cf::GPB;
+    // the transformed columns (with passC_tail, each run's column K - 1 is
+    // written by the block holding its column 0)
+    const int cols = passC_tail_on<NP>() ? d.K - 1 : d.K;
+    const int blocks = (cols * c->nruns + Cf::GPB - 1) / Cf::GPB;
     launch(c, k2_passC<T, NP>, dim3(blocks), dim3(2 * Cf::BLK), 0, (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
                        Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a,
                        batch_of(c));
