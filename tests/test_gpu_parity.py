@@ -198,12 +198,15 @@ def test_fused_cfl_mode(N, steps):
     s.close()
 
 
-@pytest.mark.parametrize("N,steps", [(64, 50), (96, 30), (256, 40), (512, 20)])
+@pytest.mark.parametrize("N,steps", [(64, 50), (96, 30), (256, 40), (512, 20), (1024, 10),
+                                     (2048, 3)])
 def test_fp32_error_vs_fp64_oracle(N, steps):
     """FP32 transforms, products and updates (FP64 twiddles, FP64 CFL and
     diagnostics): relative L2 error against the FP64 oracle, reported."""
+    import os
     from spectral2d_amd.spectral import FP32
     print(f"[parity] FP32 N={N} steps={steps}", flush=True)
+    oracle.set_threads(min(16, os.cpu_count() or 1))
     o = oracle.Oracle(N)
     w, psi = _turbulence(o, 6)
     dt = o.cfl_dt(psi, 0.5)
