@@ -60,10 +60,10 @@ struct WaveCfg {
     static constexpr bool OPS_EARLY_C = true;
     // pass A: FP64 loads its early-pass twiddles in one batch and the final
     // pass's per use (1.3 % less time at Np = 1536 than all in one batch at
-    // two waves); FP32 loads all per pass, which leaves registers for more
-    // waves (measured: 10 % less time for the mixed ensemble, 5 % at
-    // Np = 1536)
-    static constexpr int TW_PRE_A = sizeof(T) == 8 ? TW_EARLY : 0;
+    // two waves); FP32 loads all in one batch for the two-wave groups
+    // (Np >= 1536: 5 % less time) and per pass for the ensemble's sizes
+    // (4 % less time for the mixed ensemble than in one batch)
+    static constexpr int TW_PRE_A = sizeof(T) == 8 ? TW_EARLY : TG >= 128 ? TW_ALL : 0;
     // pass B: the lane's twiddles stay in registers for all of its row
     // transforms; FP64 has room next to the data (256 VGPRs) for the
     // early-pass ones only (measured: 9 % less time at Np = 1536 than
