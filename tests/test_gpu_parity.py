@@ -242,6 +242,41 @@ def test_fused_large_sizes(N, steps):
     s.close()
 
 
+@pytest.mark.parametrize("N,precision", [(512, "fp64"), (1024, "fp64"), (2048, "fp64"),
+                                         (1024, "fp32")])
+def test_last_column_mode(N, precision):
+    """A strong mode at kx = N/2, ky = 0, the one kept mode of the last
+    column, on top of a turbulent field: the fused kernels write that column
+    without transforming it (pass A: a constant along y; pass C from
+    Np = 1536 up: the column sum of H_uw). Against the oracle, which
+    transforms every column."""
+    import os
+    from spectral2d_amd.spectral import FP32, FP64
+    print(f"[parity] last column mode N={N} {precision}", flush=True)
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    o = oracle.Oracle(N)
+    w, psi = _turbulence(o, 4)
+    o.cview(w)[0, N // 2] += 40.0 - 25.0j
+    o.compute_psi(w, psi)
+    dt = o.cfl_dt(psi, 0.5)
+    st = o.new_state()
+    st["w"][:] = w
+    st["psi"][:] = psi
+    for _ in range(3):
+        o.step(st, 1e-4, dt)
+    s = Spectral2D(N, precision=FP64 if precision == "fp64" else FP32)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.rk3_steps(3, 1e-4, dt, 0.0)
+    got, ref = s.cview(s.get_field("w")), o.cview(st["w"])
+    e = rel_l2(got, ref)
+    em = abs(got[0, N // 2] - ref[0, N // 2]) / abs(ref[0, N // 2])
+    print(f"  rel_l2(w)={e:.3e}, mode (N/2, 0) rel err={em:.3e}")
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    assert e < tol and em < tol
+    s.close()
+
+
 def test_generic_and_compiled_kernels_agree():
     """S2D_GENERIC_KERNELS=1 selects the runtime-plan kernels for a size that
     has compiled kernels; both must give the same trajectory to rounding."""
