@@ -186,15 +186,20 @@ constexpr int early_tw_offset()
 // issued after an exchange's fence cannot be moved above it, so per-pass
 // loads wait once per pass; batched ones hold registers for the whole
 // transform.
-constexpr int TW_EARLY = 1, TW_FINAL = 2, TW_ALL = 3;
+constexpr int TW_EARLY = 1, TW_FINAL = 2, TW_ALL = 3, TW_FINAL_POW = 4;
 
 template <typename T, int E, int TG, int PRE = TW_ALL>
 struct GroupTw {
     static constexpr int n = E * TG;
     static constexpr int NE = early_tw_offset<E, TG, TG>();
     static constexpr bool PE = (PRE & TW_EARLY) != 0, PF = (PRE & TW_FINAL) != 0;
+    // bit 2: the final pass's twiddles W^(m t) as powers of W^t, one complex
+    // multiplication each instead of a load (at most about E ulps apart
+    // from the table's values)
+    static constexpr bool PP = (PRE & TW_FINAL_POW) != 0;
     cpx<T> e[PE && NE > 0 ? NE : 1];
     cpx<T> f[PF ? E : 1];
+    cpx<T> w1; // W^t (PP)
     const cpx<T> *tw;
     int t;
 
@@ -203,6 +208,7 @@ struct GroupTw {
         tw = tw_;
         t = t_;
         if constexpr (PE) load_early<1>();
+        if constexpr (PP) w1 = tw[t];
         if constexpr (PF) {
 #pragma unroll
             for (int m = 1; m < E; ++m) f[m] = tw[m * t];
@@ -294,8 +300,18 @@ __device__ __forceinline__ void early_passes(cpx<T> (&v)[E], cpx<T> *lds, int t,
 template <typename T, int E, int TG, int D, class G>
 __device__ __forceinline__ void final_pass(cpx<T> (&v)[E], const G &g)
 {
+    if constexpr (G::PP) {
+        cpx<T> p = g.w1;
+        v[1] = mul(v[1], dir<T, D>(p));
 #pragma unroll
-    for (int m = 1; m < E; ++m) v[m] = mul(v[m], dir<T, D>(g.final(m)));
+        for (int m = 2; m < E; ++m) {
+            p = mul(p, g.w1);
+            v[m] = mul(v[m], dir<T, D>(p));
+        }
+    } else {
+#pragma unroll
+        for (int m = 1; m < E; ++m) v[m] = mul(v[m], dir<T, D>(g.final(m)));
+    }
     dft_ct<T, E, D>(v);
 }
 

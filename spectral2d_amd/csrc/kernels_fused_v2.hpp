@@ -58,17 +58,19 @@ struct WaveCfg {
     // ahead of the transform
     static constexpr int TW_PRE_C = TW_ALL;
     static constexpr bool OPS_EARLY_C = true;
-    // pass A: FP64 loads its early-pass twiddles in one batch and the final
-    // pass's per use (1.3 % less time at Np = 1536 than all in one batch at
-    // two waves); FP32 loads all in one batch for the two-wave groups
+    // pass A: FP64 loads its early-pass twiddles in one batch (1.3 % less
+    // time at Np = 1536 than all in one batch at two waves) and forms the
+    // final pass's as powers of W^t (6 % less time than loading them per
+    // use); FP32 loads all in one batch for the two-wave groups
     // (Np >= 1536: 5 % less time) and per pass for the ensemble's sizes
     // (4 % less time for the mixed ensemble than in one batch)
-    static constexpr int TW_PRE_A = sizeof(T) == 8 ? TW_EARLY : TG >= 128 ? TW_ALL : 0;
+    static constexpr int TW_PRE_A = sizeof(T) == 8 ? (TW_EARLY | TW_FINAL_POW) : TG >= 128 ? TW_ALL : 0;
     // pass B: the lane's twiddles stay in registers for all of its row
     // transforms; FP64 has room next to the data (256 VGPRs) for the
     // early-pass ones only (measured: 9 % less time at Np = 1536 than
-    // loading every twiddle per pass)
-    static constexpr int TW_PRE_B = sizeof(T) == 4 ? TW_ALL : TW_EARLY;
+    // loading every twiddle per pass), and forms the final pass's as powers
+    // of W^t (5 % less time than loading them)
+    static constexpr int TW_PRE_B = sizeof(T) == 4 ? TW_ALL : (TW_EARLY | TW_FINAL_POW);
     // pass B holds w and u + i v; a multi-wave group needs more registers
     static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
