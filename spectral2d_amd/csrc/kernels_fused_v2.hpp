@@ -62,9 +62,12 @@ struct WaveCfg {
     // time at Np = 1536 than all in one batch at two waves) and forms the
     // final pass's as powers of W^t (6 % less time than loading them per
     // use); FP32 loads all in one batch for the two-wave groups
-    // (Np >= 1536: 5 % less time) and per pass for the ensemble's sizes
-    // (4 % less time for the mixed ensemble than in one batch)
-    static constexpr int TW_PRE_A = sizeof(T) == 8 ? (TW_EARLY | TW_FINAL_POW) : TG >= 128 ? TW_ALL : 0;
+    // (Np >= 1536: 5 % less time), and for the ensemble's sizes loads the
+    // early-pass ones per pass and forms the final pass's as powers (4 %
+    // less time for the mixed ensemble than all in one batch, then 2.6 %
+    // less than loading the final ones too)
+    static constexpr int TW_PRE_A =
+        sizeof(T) == 8 ? (TW_EARLY | TW_FINAL_POW) : TG >= 128 ? TW_ALL : TW_FINAL_POW;
     // pass B: the lane's twiddles stay in registers for all of its row
     // transforms; FP64 has room next to the data (256 VGPRs) for the
     // early-pass ones only (measured: 9 % less time at Np = 1536 than
