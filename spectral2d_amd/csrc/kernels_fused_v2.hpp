@@ -53,11 +53,14 @@ struct WaveCfg {
     // of it without spills; FP64 pass A fits three waves in 168 by loading
     // only the early-pass twiddles in one batch)
     static constexpr int WPE_A = sizeof(T) == 8 ? 3 : 4;
-    static constexpr int WPE_C = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int WPE_C = sizeof(T) == 8 ? 2 : TG <= 64 ? 5 : 4;
     // pass C: the compact-state operands loaded ahead of the transform, and
-    // the twiddles in one batch; FP64 forms the final pass's as powers of
-    // W^t (168 VGPRs, three waves per SIMD: 7 % less time at Np = 1536)
-    static constexpr int TW_PRE_C = sizeof(T) == 8 ? (TW_EARLY | TW_FINAL_POW) : TW_ALL;
+    // the twiddles in one batch; FP64 and the ensemble's FP32 sizes form the
+    // final pass's as powers of W^t (FP64: 168 VGPRs, three waves per SIMD,
+    // 7 % less time at Np = 1536; FP32 Np <= 768: five waves, 7 % less
+    // time for the mixed ensemble)
+    static constexpr int TW_PRE_C =
+        sizeof(T) == 8 || TG <= 64 ? (TW_EARLY | TW_FINAL_POW) : TW_ALL;
     static constexpr bool OPS_EARLY_C = true;
     // pass A: FP64 loads its early-pass twiddles in one batch (1.3 % less
     // time at Np = 1536 than all in one batch at two waves) and forms the
