@@ -233,9 +233,10 @@ def run_c1_fp32(N: int, steps: int, warmup: int, nu: float, cfl: float):
 
 
 def run_host_exchange(N: int, steps: int, nu: float, cfl: float):
-    """Steps as the Fortran IntegrateOneStep makes them by default: one step,
-    then wC and psiC copied to host buffers in the reference layout (the
-    only fields valid after a step)."""
+    """Steps as the Fortran IntegrateOneStep makes them by default: the host
+    wC uploaded (the host arrays are the state), one step, then wC and psiC
+    copied back to host buffers in the reference layout (the only fields
+    valid after a step)."""
     import numpy as np
     from spectral2d_amd.spectral import FP64, Spectral2D
     sim = Spectral2D(N, precision=FP64)
@@ -244,17 +245,20 @@ def run_host_exchange(N: int, steps: int, nu: float, cfl: float):
     w = np.empty(2 * sim.cISize * sim.nyG)
     p = np.empty_like(w)
     sim.rk3_steps(2, nu, 0.0, cfl)
+    sim.get_field("w", out=w)
     t0 = time.perf_counter()
     for _ in range(steps):
+        sim.set_field("w", w)
         sim.rk3_steps(1, nu, 0.0, cfl)
         sim.get_field("w", out=w)
         sim.get_field("psi", out=p)
     el = time.perf_counter() - t0
     sim.close()
-    return {"metric": "Mpts·RK3-steps/s with wC, psiC copied to the host after every step",
+    return {"metric": "Mpts·RK3-steps/s with wC uploaded before and wC, psiC copied back "
+                      "after every step",
             "value": round(N * N / 1e6 * steps / el, 3), "unit": UNIT, "steps": steps,
             "ms_per_step": round(el / steps * 1e3, 5), "dtype": "f64",
-            "host_bytes_per_step": 2 * w.nbytes}
+            "host_bytes_per_step": 3 * w.nbytes}
 
 
 def run_closure(members: int):

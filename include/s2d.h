@@ -182,6 +182,11 @@ int s2d_diagnostics(s2d_ctx *ctx, s2d_diag *out);
 int s2d_spectrum_bins(const s2d_grid *g);
 int s2d_spectrum(s2d_ctx *ctx, double nu, double *ek, int *cnt, double *tke, double *eps,
                  double *ediss);
+/* ComputeSpectrum(..., uC, vC, tke, eps, enstDiss), src/Analysis.F90:267-460,
+ * from the velocity spectra held in slots u and v (Ehat = |u|^2 + |v|^2 of
+ * each stored mode, src/Analysis.F90:384), with the outputs of s2d_spectrum. */
+int s2d_spectrum_uv(s2d_ctx *ctx, int u, int v, double nu, double *ek, int *cnt, double *tke,
+                    double *eps, double *ediss);
 /* Simulation time / step count held by the context (advanced by the step
  * calls); s2d_set_time resets them. */
 int s2d_set_time(s2d_ctx *ctx, double time, long long nadv);
@@ -213,12 +218,16 @@ int s2d_clear_forcing(s2d_ctx *ctx);
  * slot. */
 struct s2d_cnn;
 int s2d_closure_forcing(s2d_ctx *ctx, s2d_ctx *grid, struct s2d_cnn *net);
-/* nsteps fused CFL steps (cfl > 0) of a closure run, the forcing refreshed
- * at every `every`-th step: one step (with or without the refresh) is
- * captured once into a hipGraph and replayed (re-captured when the grid,
- * network, nu or cfl change). Checked for NaN/Inf at the end. */
+/* nsteps fused steps of a closure run, the forcing refreshed (as
+ * s2d_closure_forcing) before a step once `every` steps have been taken with
+ * the current one; the count runs on across calls and is reset by
+ * s2d_closure_forcing / s2d_set_forcing, so chunks of CFL steps and single
+ * fixed steps keep one refresh period. cfl > 0: CFL steps, each step (with
+ * or without the refresh) captured once into a hipGraph and replayed
+ * (re-captured when the grid, network, nu or cfl change); cfl <= 0: fixed
+ * steps of dt, launched eagerly. Checked for NaN/Inf at the end. */
 int s2d_closure_steps(s2d_ctx *ctx, s2d_ctx *grid, struct s2d_cnn *net, int nsteps, double nu,
-                      double cfl, int every);
+                      double dt, double cfl, int every);
 /* on != 0: the closure forcing is zeroed on every shell floor(|k| + 1/2) =
  * b where -F would raise the shell's enstrophy (sum over the shell of
  * Re(conj(w_hat) F) < 0): the closure keeps its net drain, shell by shell. */

@@ -44,7 +44,7 @@ EXPORTS = (
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
     "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_get_time", "s2d_profile_begin",
     "s2d_profile_every", "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
-    "s2d_spectrum_bins", "s2d_spectrum", "s2d_ensemble_spectrum",
+    "s2d_spectrum_bins", "s2d_spectrum", "s2d_spectrum_uv", "s2d_ensemble_spectrum",
     "s2d_checkpoint_write", "s2d_checkpoint_read", "s2d_restrict", "s2d_axpby",
     "s2d_set_forcing", "s2d_clear_forcing",
     "s2d_cnn_last_error", "s2d_cnn_param_count", "s2d_cnn_create", "s2d_cnn_destroy",
@@ -150,6 +150,7 @@ def lib():
     L.s2d_checkpoint_write.argtypes = [vp, C.c_char_p]
     L.s2d_checkpoint_read.argtypes = [vp, C.c_char_p]
     L.s2d_spectrum.argtypes = [vp, C.c_double, dp, ip, dp, dp, dp]
+    L.s2d_spectrum_uv.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, ip, dp, dp, dp]
     L.s2d_ensemble_spectrum.argtypes = [vp, C.c_int, C.c_double, dp, ip, dp, dp, dp]
     L.s2d_ensemble_last_error.argtypes = [vp]
     L.s2d_ensemble_last_error.restype = C.c_char_p
@@ -187,7 +188,8 @@ def lib():
     L.s2d_cnn_grid.argtypes = [vp, ip, ip]
     L.s2d_closure_forcing.argtypes = [vp, vp, vp]
     L.s2d_set_closure_guard.argtypes = [vp, C.c_int]
-    L.s2d_closure_steps.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_int]
+    L.s2d_closure_steps.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_int]
     for name in EXPORTS:
         if name == "s2d_cnn_param_count":
             L.s2d_cnn_param_count.restype = C.c_longlong

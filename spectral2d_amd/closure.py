@@ -112,16 +112,17 @@ class ClosedRun:
         g.restrict_to("tmp", s, "tmp", kc)
         s.set_forcing("tmp")
 
-    def steps(self, n: int, nu: float, cfl: float = 0.5):
-        """n CFL steps; with a network, the forcing is refreshed every
-        `every` steps and the steps replay a captured hipGraph
-        (s2d_closure_steps)."""
+    def steps(self, n: int, nu: float, cfl: float = 0.5, dt: float = 0.0):
+        """n steps, CFL (cfl > 0) or of fixed dt; with a network the
+        forcing is refreshed once `every` steps have been taken with it, a
+        period that runs on across calls (s2d_closure_steps; CFL steps replay
+        a captured hipGraph)."""
         from . import _lib as L
         if self.net is None:
-            self.sim.rk3_steps(n, nu, 0.0, cfl)
+            self.sim.rk3_steps(n, nu, dt, cfl)
         else:
             L.check(L.lib().s2d_closure_steps(self.sim._h, self.grid._h, self.net._h, int(n),
-                                              float(nu), float(cfl), int(self.every)),
+                                              float(nu), float(dt), float(cfl), int(self.every)),
                     self.sim._h)
 
     def advance_to(self, t_end: float, nu: float, cfl: float = 0.5, chunk: int = 100,
@@ -140,9 +141,7 @@ class ClosedRun:
                 raise RuntimeError(f"enstrophy grew {z / z0:.2f}x by t = {s.get_time()[0]:.4f}")
 
         def fixed_step(h):
-            if self.net is not None:
-                self.closure_forcing()
-            s.rk3_steps(1, nu, h, 0.0)
+            self.steps(1, nu, cfl=0.0, dt=h)
 
         n = _advance(s, t_end, cfl, chunk, max_steps, cfl_steps, fixed_step)
         if log:
@@ -154,13 +153,16 @@ def _advance(sim: Spectral2D, t_end: float, cfl: float, chunk: int, max_steps: i
              fixed_step) -> int:
     """Chunks of CFL steps while a chunk stays short of t_end (the chunk
     halved as t_end nears, a step taken as at most 1.25 times the last
-    dt), then fixed steps of min(CFL dt, t_end - t) that land on t_end."""
+    dt), then fixed steps of min(CFL dt, t_end - t) that land on t_end.
+    A chunk whose steps grew faster than that and passed t_end raises
+    RuntimeError (the comparison at t_end would be made at a later time)."""
     n = 0
     t, _, dt = sim.get_time()
     if dt <= 0.0:
         dt = sim.ComputeTimeStep(cfl)
     c = chunk
-    while t < t_end - 1e-12:
+    tol = 1e-12 * max(1.0, abs(t_end))
+    while t < t_end - tol:
         if n >= max_steps:
             raise RuntimeError(f"no arrival at t = {t_end} within {max_steps} steps (t = {t})")
         while c > 1 and t + c * 1.25 * dt >= t_end:
@@ -172,6 +174,11 @@ def _advance(sim: Spectral2D, t_end: float, cfl: float, chunk: int, max_steps: i
             fixed_step(min(sim.ComputeTimeStep(cfl), t_end - t))
             n += 1
         t, _, dt = sim.get_time()
+        if t > t_end + tol:
+            raise RuntimeError(f"a chunk of CFL steps passed t_end = {t_end} (t = {t}): dt "
+                               f"grew faster than 1.25x per step")
+    if abs(t - t_end) > tol:
+        raise RuntimeError(f"stopped at t = {t}, not t_end = {t_end}")
     return n
 
 

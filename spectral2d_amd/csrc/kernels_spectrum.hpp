@@ -36,11 +36,22 @@ __device__ __forceinline__ double mode_energy(cpx<T> p, int kx, int ky, bool mx,
     return (ur * ur + ui * ui) + (vr * vr + vi * vi);
 }
 
-// ek[b] for b in [0, nbins): psi in the reference layout (cISize, nyG),
-// FFTW-order rows. kxMG / kyMG: the Nyquist wavenumbers (masked and weight
-// 1/2 at kx = 0 and kx = kxMG, src/Analysis.F90:389-395).
+// Ehat of one mode from the velocity spectra themselves, as
+// ComputeSpectrum forms it from its uC, vC arguments (src/Analysis.F90:384).
 template <typename T>
-__global__ __launch_bounds__(256) void k_spectrum(const cpx<T> *__restrict__ psi, int cISize,
+__device__ __forceinline__ double mode_energy_uv(cpx<T> u, cpx<T> v)
+{
+    const double ur = (double)u.re, ui = (double)u.im, vr = (double)v.re, vi = (double)v.im;
+    return (ur * ur + ui * ui) + (vr * vr + vi * vi);
+}
+
+// ek[b] for b in [0, nbins), in the reference layout (cISize, nyG) with
+// FFTW-order rows: from psi (UV false; v unused) or from the velocity
+// spectra u, v (UV true). kxMG / kyMG: the Nyquist wavenumbers (masked and
+// weight 1/2 at kx = 0 and kx = kxMG, src/Analysis.F90:389-395).
+template <typename T, bool UV = false>
+__global__ __launch_bounds__(256) void k_spectrum(const cpx<T> *__restrict__ psi,
+                                                  const cpx<T> *__restrict__ v, int cISize,
                                                   int nyG, int kxMG, int kyMG, int maskX,
                                                   int maskY, double *__restrict__ ek)
 {
@@ -63,7 +74,11 @@ __global__ __launch_bounds__(256) void k_spectrum(const cpx<T> *__restrict__ psi
                 if (ky > nyG / 2 || ky < -(nyG / 2) + 1) continue;
                 const int j = (int)(ky >= 0 ? ky : nyG + ky);
                 const bool my = maskY && ky == kyMG;
-                acc += wt * mode_energy(psi[(size_t)kx + (size_t)j * cISize], kx, (int)ky, mx, my);
+                const size_t o = (size_t)kx + (size_t)j * cISize;
+                if constexpr (UV)
+                    acc += wt * mode_energy_uv(psi[o], v[o]);
+                else
+                    acc += wt * mode_energy(psi[o], kx, (int)ky, mx, my);
             }
         }
     }

@@ -863,8 +863,10 @@ int diag_of(s2d_ctx *c, int run, s2d_diag *out)
 
 // ComputeSpectrum of the PSI slot (brought up to date by the caller):
 // device shell sums, host mode counts and moments in shell order.
+// The shell spectrum of the psi slot fa (fb < 0), or of the velocity
+// spectra in slots fa (u) and fb (v).
 int spectrum_of(s2d_ctx *c, double nu, double *ek, int *cnt, double *tke, double *eps,
-                double *ediss)
+                double *ediss, int fa, int fb)
 {
     const s2d_grid &g = c->g;
     const int nb = s2d_spectrum_bins(&g);
@@ -875,14 +877,26 @@ int spectrum_of(s2d_ctx *c, double nu, double *ek, int *cnt, double *tke, double
         S2D_HIP(c, hipMalloc((void **)&own, sizeof(double) * nb));
         d_ek = own;
     }
-    if (c->precision == S2D_FP64)
-        hipLaunchKernelGGL(k_spectrum<double>, dim3(nb), dim3(256), 0, c->stream,
-                           F<double>(c, S2D_FIELD_PSI), g.cISize, g.nyG, g.kxMG, g.kyMG,
-                           c->xMaskBool, c->yMaskBool, d_ek);
-    else
-        hipLaunchKernelGGL(k_spectrum<float>, dim3(nb), dim3(256), 0, c->stream,
-                           F<float>(c, S2D_FIELD_PSI), g.cISize, g.nyG, g.kxMG, g.kyMG,
-                           c->xMaskBool, c->yMaskBool, d_ek);
+    const int fv = fb < 0 ? fa : fb;
+    if (c->precision == S2D_FP64) {
+        if (fb < 0)
+            hipLaunchKernelGGL((k_spectrum<double, false>), dim3(nb), dim3(256), 0, c->stream,
+                               F<double>(c, fa), F<double>(c, fv), g.cISize, g.nyG, g.kxMG,
+                               g.kyMG, c->xMaskBool, c->yMaskBool, d_ek);
+        else
+            hipLaunchKernelGGL((k_spectrum<double, true>), dim3(nb), dim3(256), 0, c->stream,
+                               F<double>(c, fa), F<double>(c, fv), g.cISize, g.nyG, g.kxMG,
+                               g.kyMG, c->xMaskBool, c->yMaskBool, d_ek);
+    } else {
+        if (fb < 0)
+            hipLaunchKernelGGL((k_spectrum<float, false>), dim3(nb), dim3(256), 0, c->stream,
+                               F<float>(c, fa), F<float>(c, fv), g.cISize, g.nyG, g.kxMG,
+                               g.kyMG, c->xMaskBool, c->yMaskBool, d_ek);
+        else
+            hipLaunchKernelGGL((k_spectrum<float, true>), dim3(nb), dim3(256), 0, c->stream,
+                               F<float>(c, fa), F<float>(c, fv), g.cISize, g.nyG, g.kxMG,
+                               g.kyMG, c->xMaskBool, c->yMaskBool, d_ek);
+    }
     const hipError_t el = hipGetLastError();
     std::vector<double> h((size_t)nb);
     const hipError_t ec =
@@ -1625,7 +1639,14 @@ int s2d_spectrum(s2d_ctx *c, double nu, double *ek, int *cnt, double *tke, doubl
     S2D_TRY(check_ctx(c));
     S2D_TRY(set_device(c));
     S2D_TRY(ensure_ref(c));
-    return spectrum_of(c, nu, ek, cnt, tke, eps, ediss);
+    return spectrum_of(c, nu, ek, cnt, tke, eps, ediss, S2D_FIELD_PSI, -1);
+}
+
+int s2d_spectrum_uv(s2d_ctx *c, int u, int v, double nu, double *ek, int *cnt, double *tke,
+                    double *eps, double *ediss)
+{
+    S2D_OP_PROLOGUE(c, u, v);
+    return spectrum_of(c, nu, ek, cnt, tke, eps, ediss, u, v);
 }
 
 int s2d_restrict(s2d_ctx *src, int sf, s2d_ctx *dst, int df, double kcut)
@@ -1696,6 +1717,7 @@ int s2d_set_forcing(s2d_ctx *c, int f)
     c->Wn = keep;
     S2D_TRY(r);
     c->forcing_on = true;
+    c->cl_age = 0;
     return sync(c);
 }
 
@@ -1709,46 +1731,60 @@ int s2d_closure_forcing(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net)
     if (!c->forcing) S2D_HIP(c, hipMalloc(&c->forcing, (size_t)c->sW * c->celt));
     S2D_TRY(enqueue_closure_forcing(c, g, net, false));
     c->forcing_on = true;
+    c->cl_age = 0;
     return sync(c);
 }
 
-int s2d_closure_steps(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, int nsteps, double nu, double cfl,
-                      int every)
+int s2d_closure_steps(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, int nsteps, double nu, double dt,
+                      double cfl, int every)
 {
     S2D_TRY(check_closure(c, g, net));
     if (nsteps < 0 || every < 1) return fail(c, S2D_ERR_INVALID_ARG, "bad step count or period");
-    if (!(cfl > 0.0)) return fail(c, S2D_ERR_INVALID_ARG, "closure steps take a CFL time step");
+    if (!(cfl > 0.0) && !(dt >= 0.0)) return fail(c, S2D_ERR_INVALID_ARG, "invalid dt");
     S2D_TRY(set_device(c));
     S2D_TRY(ensure_compact(c));
     S2D_TRY(sync(g));
     if (!c->forcing) S2D_HIP(c, hipMalloc(&c->forcing, (size_t)c->sW * c->celt));
     c->forcing_on = true;
-    const bool fresh = c->cl_net == net && c->cl_grid == g && c->cl_nu == nu && c->cl_cfl == cfl;
-    if (!fresh) drop_closure_graphs(c);
-    for (int k = 0; k < 2 && !c->cl_graph[k]; ++k) {
-        // k = 0: forcing refresh + step; k = 1: step with the frozen forcing
-        hipGraph_t gr = nullptr;
-        S2D_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int r = k == 0 ? enqueue_closure_forcing(c, g, net, true) : S2D_OK;
-        if (!r) r = fused_steps<float>(c, 1, nu, 0.0, cfl);
-        const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
-        if (r || ec != hipSuccess || !gr) {
-            if (gr) (void)hipGraphDestroy(gr);
-            return r ? r : fail(c, S2D_ERR_HIP, "capture of the closure step failed");
+    if (cfl > 0.0) {
+        const bool fresh =
+            c->cl_net == net && c->cl_grid == g && c->cl_nu == nu && c->cl_cfl == cfl;
+        if (!fresh) drop_closure_graphs(c);
+        for (int k = 0; k < 2 && !c->cl_graph[k]; ++k) {
+            // k = 0: forcing refresh + step; k = 1: step with the frozen forcing
+            hipGraph_t gr = nullptr;
+            S2D_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int r = k == 0 ? enqueue_closure_forcing(c, g, net, true) : S2D_OK;
+            if (!r) r = fused_steps<float>(c, 1, nu, 0.0, cfl);
+            const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
+            if (r || ec != hipSuccess || !gr) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return r ? r : fail(c, S2D_ERR_HIP, "capture of the closure step failed");
+            }
+            const hipError_t ei = hipGraphInstantiate(&c->cl_graph[k], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ei != hipSuccess) {
+                c->cl_graph[k] = nullptr;
+                return fail(c, S2D_ERR_HIP, "hipGraphInstantiate failed: %s",
+                            hipGetErrorString(ei));
+            }
         }
-        const hipError_t ei = hipGraphInstantiate(&c->cl_graph[k], gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (ei != hipSuccess) {
-            c->cl_graph[k] = nullptr;
-            return fail(c, S2D_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
-        }
+        c->cl_net = net;
+        c->cl_grid = g;
+        c->cl_nu = nu;
+        c->cl_cfl = cfl;
     }
-    c->cl_net = net;
-    c->cl_grid = g;
-    c->cl_nu = nu;
-    c->cl_cfl = cfl;
-    for (int st = 0; st < nsteps; ++st)
-        S2D_HIP(c, hipGraphLaunch(c->cl_graph[st % every == 0 ? 0 : 1], c->stream));
+    // the refresh period runs on across calls and across CFL / fixed steps
+    for (int st = 0; st < nsteps; ++st) {
+        const bool refresh = c->cl_age >= every;
+        if (cfl > 0.0) {
+            S2D_HIP(c, hipGraphLaunch(c->cl_graph[refresh ? 0 : 1], c->stream));
+        } else {
+            if (refresh) S2D_TRY(enqueue_closure_forcing(c, g, net, true));
+            S2D_TRY(fused_steps<float>(c, 1, nu, dt, 0.0));
+        }
+        c->cl_age = refresh ? 1 : c->cl_age + 1;
+    }
     c->ref_stale = true;
     const size_t n = (size_t)c->K * c->KP;
     hipLaunchKernelGGL(k_check_finite<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
@@ -1777,6 +1813,7 @@ int s2d_clear_forcing(s2d_ctx *c)
 {
     S2D_TRY(check_ctx(c));
     c->forcing_on = false;
+    c->cl_age = 1 << 30;
     return S2D_OK;
 }
 

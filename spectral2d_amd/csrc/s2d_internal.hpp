@@ -77,6 +77,9 @@ struct s2d_ctx {
     void *cl_grid = nullptr, *cl_net = nullptr;
     double cl_nu = 0.0, cl_cfl = 0.0;
     bool closure_guard = false; // zero the closure forcing on backscattering shells
+    // steps taken with the current forcing (s2d_closure_steps refreshes it
+    // once this reaches its period); large: no forcing computed yet
+    int cl_age = 1 << 30;
 
     // fused compact state
     bool fused = false;

@@ -37,7 +37,8 @@ MODULE Spectral_m
 
 CONTAINS
 
-   !> GetGridSize, src/Spectral.F90:117-219 (host only).
+   !> GetGridSize, src/Spectral.F90:117-219 (host only): the sizes and the
+   !! FFTW-order wavenumbers from libs2d (s2d_get_grid_size, s2d_wavenumbers).
    SUBROUTINE GetGridSize(nx, ny, dealias_, nxG, nyG, &
                           kxLG, kyLG, kxMG, kyMG, &
                           kxLU, kyLU, kxMU, kyMU, &
@@ -49,46 +50,30 @@ CONTAINS
       INTEGER(KIND=IWPF),INTENT(OUT) :: rISize, cISize
       REAL(KIND=RWPF),INTENT(OUT) :: kMax
       INTEGER(KIND=IWPF),DIMENSION(:),ALLOCATABLE,INTENT(INOUT) :: kxG, kyG
-      INTEGER(KIND=IWPF) :: i
+      TYPE(s2d_grid_t) :: g
 
+      CALL S2D_Check(s2d_get_grid_size(INT(nx, C_INT), INT(ny, C_INT), INT(dealias_, C_INT), &
+                                       g), C_NULL_PTR, 'GetGridSize')
       dealias = dealias_
       nxReq = nx
       nyReq = ny
-      SELECT CASE (dealias)
-         CASE (NO_DEALIASING)
-            nxG = nx
-            nyG = ny
-         CASE (DEALIAS_3_2_RULE)
-            nxG = 3_IWPF*nx/2_IWPF
-            nyG = 3_IWPF*ny/2_IWPF
-         CASE DEFAULT
-            WRITE(*,'(A,I0,A)') 'GetGridSize: dealiasing method ', dealias, &
-                                ' is not defined'
-            ERROR STOP 1
-      END SELECT
-      kxLG = 0_IWPF
-      kyLG = -nyG/2_IWPF + 1_IWPF
-      kxMG = nxG/2_IWPF
-      kyMG = nyG/2_IWPF
-      kxLU = 0_IWPF
-      kyLU = -ny/2_IWPF + 1_IWPF
-      kxMU = nx/2_IWPF
-      kyMU = ny/2_IWPF
-      kMax = REAL(kxMU, RWPF)
-      rISize = 2_IWPF*(nxG/2_IWPF + 1_IWPF)
-      cISize = nxG/2_IWPF + 1_IWPF
-      ALLOCATE(kxG(cISize))
-      DO i = 1, cISize
-         kxG(i) = i - 1_IWPF
-      END DO
-      ALLOCATE(kyG(nyG))
-      DO i = 1, nyG
-         IF (i - 1_IWPF > nyG/2_IWPF) THEN
-            kyG(i) = i - 1_IWPF - nyG
-         ELSE
-            kyG(i) = i - 1_IWPF
-         END IF
-      END DO
+      nxG = g%nxG
+      nyG = g%nyG
+      kxLG = g%kxLG
+      kyLG = g%kyLG
+      kxMG = g%kxMG
+      kyMG = g%kyMG
+      kxLU = g%kxLU
+      kyLU = g%kyLU
+      kxMU = g%kxMU
+      kyMU = g%kyMU
+      kMax = REAL(g%kMax, RWPF)
+      rISize = g%rISize
+      cISize = g%cISize
+      IF (ALLOCATED(kxG)) DEALLOCATE(kxG)
+      IF (ALLOCATED(kyG)) DEALLOCATE(kyG)
+      ALLOCATE(kxG(cISize), kyG(nyG))
+      CALL S2D_Check(s2d_wavenumbers(g, kxG, kyG), C_NULL_PTR, 'GetGridSize')
    END SUBROUTINE GetGridSize
 
    !> SetupMask, src/Spectral.F90:229-260: the masks are computed by

@@ -3,13 +3,17 @@
 !! module (src/TimeIntegration.F90:67-68), running the fused TVD-RK3 step of
 !! libs2d.
 !!
-!! The vorticity lives on the GPU between steps. IntegrateOneStep uploads
-!! wC when the host copy is newer (after setup, SetIC or a restart read;
-!! MarkHostModified signals it) and, unless the caller has switched to
-!! device-resident mode, downloads wC and psiC after the step so that, as in
-!! the reference, only wC and psiC are valid afterwards
-!! (src/Spectral2D.F90:341). In device-resident mode the host copies are
-!! refreshed by SyncHostFields only (e.g. at the print/IO cadence).
+!! Default mode: the host wC and psiC are the state, as in the reference.
+!! IntegrateOneStep uploads them, steps on the GPU and downloads them, so
+!! that only wC and psiC are valid afterwards (src/Spectral2D.F90:341) and
+!! whatever the caller wrote into them between steps (the reference's
+!! WriteRestart round trip, src/IO.F90:118-142, a restart read, SetIC) is
+!! what the next step advances.
+!!
+!! Device-resident mode (SetDeviceResident): the state stays on the GPU
+!! between steps; the host copies are refreshed by SyncHostFields only, and
+!! a host change reaches the GPU only through UploadHostFields or
+!! MarkHostModified (the next step then uploads).
 MODULE TimeIntegration_m
 
    USE ISO_C_BINDING
@@ -61,6 +65,10 @@ CONTAINS
          CALL S2D_Check(s2d_set_field_host(s2dCtx, S2D_FIELD_PSI, psiC), s2dCtx, &
                         'IntegrateOneStep')
          hostNewer = .FALSE.
+      ELSE IF (.NOT. deviceResident) THEN
+         ! the fused step derives psi from w (psi_hat = w_hat/|k|^2), so the
+         ! host w is the whole input
+         CALL S2D_Check(s2d_set_field_host(s2dCtx, S2D_FIELD_W, wC), s2dCtx, 'IntegrateOneStep')
       END IF
       CALL S2D_Check(s2d_rk3_steps(s2dCtx, 1_C_INT, REAL(nu, C_DOUBLE), REAL(dt, C_DOUBLE), &
                                    0.0_C_DOUBLE), s2dCtx, 'IntegrateOneStep')
@@ -110,10 +118,11 @@ CONTAINS
       hostNewer = .FALSE.
    END SUBROUTINE UploadHostFields
 
-   !> Copy the device state into wC and psiC.
+   !> Copy the device state into wC and psiC (device-resident mode; in the
+   !! default mode the host arrays already are the state).
    SUBROUTINE SyncHostFields(wC, psiC)
       COMPLEX(KIND=CWPC),DIMENSION(:,:),INTENT(INOUT) :: wC, psiC
-      IF (hostNewer) RETURN
+      IF (hostNewer .OR. .NOT. deviceResident) RETURN
       CALL S2D_Check(s2d_get_field_host(s2dCtx, S2D_FIELD_W, wC), s2dCtx, 'SyncHostFields')
       CALL S2D_Check(s2d_get_field_host(s2dCtx, S2D_FIELD_PSI, psiC), s2dCtx, 'SyncHostFields')
    END SUBROUTINE SyncHostFields

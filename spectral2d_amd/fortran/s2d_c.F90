@@ -29,6 +29,17 @@ MODULE S2D_C
       INTEGER(C_LONG_LONG) :: nadv
    END TYPE
 
+   !> Mirror of s2d_grid (GetGridSize + GetAllocSize with one process).
+   TYPE,BIND(C) :: s2d_grid_t
+      INTEGER(C_INT) :: nx, ny, dealias
+      INTEGER(C_INT) :: nxG, nyG
+      INTEGER(C_INT) :: kxLG, kyLG, kxMG, kyMG
+      INTEGER(C_INT) :: kxLU, kyLU, kxMU, kyMU
+      REAL(C_DOUBLE) :: kMax
+      INTEGER(C_INT) :: rISize, cISize
+      INTEGER(C_INT) :: nxP, nyP, j1, j2, localJ, allocLocal
+   END TYPE
+
    INTERFACE
       FUNCTION s2d_last_error(ctx) BIND(C, name='s2d_last_error')
          IMPORT :: C_PTR
@@ -138,6 +149,57 @@ MODULE S2D_C
          TYPE(C_PTR),VALUE :: ctx
          INTEGER(C_LONG_LONG),VALUE :: seed
          REAL(C_DOUBLE),VALUE :: k0, ke
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_get_grid_size(nx, ny, dealias, g) &
+            BIND(C, name='s2d_get_grid_size')
+         IMPORT :: C_INT, s2d_grid_t
+         INTEGER(C_INT),VALUE :: nx, ny, dealias
+         TYPE(s2d_grid_t) :: g
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_wavenumbers(g, kxG, kyG) BIND(C, name='s2d_wavenumbers')
+         IMPORT :: C_INT, s2d_grid_t
+         TYPE(s2d_grid_t) :: g
+         INTEGER(C_INT),DIMENSION(*) :: kxG, kyG
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_spectrum_bins(g) BIND(C, name='s2d_spectrum_bins')
+         IMPORT :: C_INT, s2d_grid_t
+         TYPE(s2d_grid_t) :: g
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_spectrum_uv(ctx, u, v, nu, ek, cnt, tke, eps, ediss) &
+            BIND(C, name='s2d_spectrum_uv')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR),VALUE :: ctx
+         INTEGER(C_INT),VALUE :: u, v
+         REAL(C_DOUBLE),VALUE :: nu
+         REAL(C_DOUBLE),DIMENSION(*) :: ek
+         INTEGER(C_INT),DIMENSION(*) :: cnt
+         REAL(C_DOUBLE) :: tke, eps, ediss
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_get_grid(ctx, g) BIND(C, name='s2d_get_grid')
+         IMPORT :: C_INT, C_PTR, s2d_grid_t
+         TYPE(C_PTR),VALUE :: ctx
+         TYPE(s2d_grid_t) :: g
+      END FUNCTION
+      ! libs2d_h5 (include/s2d_h5.h): the reference's output files
+      INTEGER(C_INT) FUNCTION s2d_h5_write_rest(path, nxG, nyG, rISize, nadv, time, nu, &
+                                                omega, psi, u, v) &
+            BIND(C, name='s2d_h5_write_rest')
+         IMPORT :: C_INT, C_CHAR, C_DOUBLE
+         CHARACTER(KIND=C_CHAR),DIMENSION(*) :: path
+         INTEGER(C_INT),VALUE :: nxG, nyG, rISize, nadv
+         REAL(C_DOUBLE),VALUE :: time, nu
+         REAL(C_DOUBLE),DIMENSION(*) :: omega, psi, u, v
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_h5_write_grid(path, nxG, nyG) BIND(C, name='s2d_h5_write_grid')
+         IMPORT :: C_INT, C_CHAR
+         CHARACTER(KIND=C_CHAR),DIMENSION(*) :: path
+         INTEGER(C_INT),VALUE :: nxG, nyG
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION s2d_h5_write_xmf(path, nxG, nyG, num) &
+            BIND(C, name='s2d_h5_write_xmf')
+         IMPORT :: C_INT, C_CHAR
+         CHARACTER(KIND=C_CHAR),DIMENSION(*) :: path
+         INTEGER(C_INT),VALUE :: nxG, nyG, num
       END FUNCTION
       FUNCTION c_strlen(s) BIND(C, name='strlen')
          IMPORT :: C_PTR, C_SIZE_T

@@ -215,6 +215,19 @@ class Spectral2D:
                                      C.byref(t), C.byref(e), C.byref(z)))
         return ek, cnt, t.value, e.value, z.value
 
+    def ComputeSpectrumUV(self, u, v, nu: float = 0.0):
+        """ComputeSpectrum from the velocity spectra in slots u and v
+        (src/Analysis.F90:267-460, its uC, vC arguments): (ek, cnt, tke, eps,
+        enstrophy_dissipation), FP64 on the device."""
+        nb = L.lib().s2d_spectrum_bins(C.byref(self.g))
+        ek = np.zeros(nb, np.float64)
+        cnt = np.zeros(nb, np.int32)
+        t, e, z = C.c_double(), C.c_double(), C.c_double()
+        self._c(L.lib().s2d_spectrum_uv(self._h, _fid(u), _fid(v), float(nu), _dp(ek),
+                                        cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                        C.byref(t), C.byref(e), C.byref(z)))
+        return ek, cnt, t.value, e.value, z.value
+
     def restrict_to(self, f, dst: "Spectral2D", df, kcut: float):
         """dst's slot df = the modes |k| <= kcut of this context's slot f
         (sharp spectral cutoff, sampled on dst's grid)."""

@@ -15,23 +15,32 @@ from spectral2d_amd.spectral import GetGridSize, Wavenumbers
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = set()
-    for fn in os.listdir(os.path.join(ROOT, "include")):
-        if fn.endswith(".h"):
-            txt = open(os.path.join(ROOT, "include", fn)).read()
-            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-            names |= set(re.findall(r"\b(s2d_\w+)\s*\(", txt))
-    return names
+def declared_functions(header="s2d.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return set(re.findall(r"\b(s2d_\w+)\s*\(", txt))
+
+
+def test_every_header_is_checked():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
+        ["s2d.h", "s2d_h5.h"]
 
 
 def test_library_exports_every_declared_function():
     lib = C.CDLL(L.LIB_PATH)
-    decl = declared_functions()
+    decl = declared_functions("s2d.h")
     assert len(decl) >= 30
     missing = [n for n in sorted(decl) if not hasattr(lib, n)]
     assert not missing, missing
     assert decl <= set(L.EXPORTS), sorted(decl - set(L.EXPORTS))
+
+
+def test_h5_library_exports_every_declared_function():
+    from spectral2d_amd import build
+    lib = C.CDLL(build.LIB_H5)
+    decl = declared_functions("s2d_h5.h")
+    assert decl == {"s2d_h5_write_rest", "s2d_h5_write_grid", "s2d_h5_write_xmf"}
+    assert all(hasattr(lib, n) for n in decl)
 
 
 @pytest.mark.parametrize("nx,ny,d", [(96, 96, 1), (1024, 1024, 1), (64, 32, 1), (16, 16, 0)])

@@ -38,3 +38,28 @@ def test_checksum_definition():
     s2 = (1 * 1 + 2 * 2 + 3 * 3 + 4 * 0xFFFFFFFFFFFFFFFF) % 2**64
     rot = ((s2 << 32) | (s2 >> 32)) % 2**64
     assert checksum(words.tobytes()) == s1 ^ rot
+
+
+def test_forcing_section_and_version_1(tmp_path):
+    """Version 2 appends the forcing term after the compact state; a version-1
+    file (no forcing, those header bytes zero) still reads."""
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal((24, 13)) + 1j * rng.standard_normal((24, 13))
+    comp = (rng.standard_normal((9, 24)) + 1j * rng.standard_normal((9, 24))).astype(np.complex64)
+    forc = (rng.standard_normal((9, 24)) + 1j * rng.standard_normal((9, 24))).astype(np.complex64)
+    f = tmp_path / "f.ckpt"
+    write_checkpoint(str(f), _hdr(precision=1), w, w, comp, forc)
+    d = read_checkpoint(str(f))
+    assert d["version"] == 2
+    assert np.array_equal(d["compact"], comp) and np.array_equal(d["forcing"], forc)
+    raw = bytearray(f.read_bytes())
+    assert raw[96:100] == b"\x01\x00\x00\x00"
+    # the same state without forcing, relabelled version 1
+    write_checkpoint(str(f), _hdr(precision=1), w, w, comp)
+    raw = bytearray(f.read_bytes())
+    raw[8:12] = (1).to_bytes(4, "little")
+    f.write_bytes(bytes(raw))
+    d = read_checkpoint(str(f))
+    assert d["version"] == 1 and d["forcing"] is None and np.array_equal(d["compact"], comp)
+    with pytest.raises(ValueError):
+        write_checkpoint(str(f), _hdr(), w, w, None, forc)

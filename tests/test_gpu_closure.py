@@ -62,26 +62,36 @@ def test_zero_closure_is_the_plain_run():
 
 def test_captured_closure_steps_equal_eager_steps():
     """s2d_closure_steps (hipGraph replay, forcing refreshed every 2 steps)
-    equals the same sequence of eager calls bitwise."""
+    equals the same sequence of eager calls bitwise; the refresh period runs
+    on across calls and into fixed-dt steps (steps 0, 2, 4, ... refresh)."""
     N = 128
     print("[closure] captured closure steps vs eager", flush=True)
     net = _net(N)
     a, b = ClosedRun(N, net, every=2), ClosedRun(N, net, every=2)
     for r in (a, b):
         r.sim.SetTurbulence(13, 8.0, 0.5)
+    k = 0
+
+    def eager(n, cfl=0.5, dt=0.0):
+        nonlocal k
+        for _ in range(n):
+            if k % 2 == 0:
+                b.closure_forcing()
+            b.sim.rk3_steps(1, 1e-4, dt, cfl)
+            k += 1
+
     a.steps(5, 1e-4, 0.5)
-    for k in range(5):
-        if k % 2 == 0:
-            b.closure_forcing()
-        b.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+    eager(5)
     assert np.array_equal(a.sim.get_field("w"), b.sim.get_field("w"))
     assert a.sim.get_time() == b.sim.get_time()
     a.steps(3, 1e-4, 0.5)  # replays the cached graphs
-    for k in range(3):
-        if k % 2 == 0:
-            b.closure_forcing()
-        b.sim.rk3_steps(1, 1e-4, 0.0, 0.5)
+    eager(3)
     assert np.array_equal(a.sim.get_field("w"), b.sim.get_field("w"))
+    h = 0.5 * a.sim.get_time()[2]
+    a.steps(3, 1e-4, cfl=0.0, dt=h)  # fixed steps keep the period
+    eager(3, cfl=0.0, dt=h)
+    assert np.array_equal(a.sim.get_field("w"), b.sim.get_field("w"))
+    assert a.sim.get_time() == b.sim.get_time()
     for r in (a, b):
         r.close()
     net.close()

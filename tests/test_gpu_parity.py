@@ -420,3 +420,99 @@ def test_rectangular_grids(nx, ny):
     assert np.array_equal(s2.get_field("w"), s.get_field("w"))
     s.close()
     s2.close()
+
+
+# ---- the per-routine kernels at the full sizes the reference's IO and
+# Analysis call them on (src/IO.F90:118-142, src/Analysis.F90:142-231):
+# Np = 1536 (1024^2) and 3072 (2048^2), where the column and row kernels
+# size their batches from the LDS budget.
+
+@pytest.mark.parametrize("N", [1024, 2048])
+def test_transforms_full_size(N):
+    oracle.set_threads(8)
+    o, s = _pair(N)
+    rng = np.random.default_rng(N)
+    tol = 1e-14 * np.log2(o.nxG * o.nyG)
+    buf = _random_spectral(o, N)
+    ref = buf.copy()
+    o.transform_c2r(ref)
+    s.set_field("tmp", buf)
+    s.TransformC2R("tmp")
+    got = s.get_field("tmp")
+    e = rel_l2(s.rview(got)[:, : o.nxG], o.rview(ref)[:, : o.nxG])
+    print(f"  c2r N={N} rel_l2={e:.3e}", flush=True)
+    assert e < tol
+    real = rng.standard_normal((o.nyG, o.nxG))
+    buf = o.from_real(real)
+    ref = buf.copy()
+    o.transform_r2c(ref)
+    s.set_field("tmp", buf)
+    s.TransformR2C("tmp")
+    e = rel_l2(s.cview(s.get_field("tmp")), o.cview(ref))
+    print(f"  r2c N={N} rel_l2={e:.3e}", flush=True)
+    assert e < tol
+    s.close()
+
+
+def test_pointwise_and_velocity_full_size():
+    """1024^2: the pointwise routines bitwise, ComputeVelocity with the
+    inverse transform (the WriteRestart form) at the rounding level."""
+    oracle.set_threads(8)
+    N = 1024
+    o, s = _pair(N)
+    a = _random_spectral(o, 3)
+    for name, fo, fs in (("dudx", o.dudx, s.DuDx), ("dudy", o.dudy, s.DuDy)):
+        ref = o.field()
+        fo(a, ref)
+        s.set_field("u", a)
+        fs("u", "v")
+        assert np.array_equal(s.get_field("v"), ref), name
+    ref = o.field()
+    o.compute_psi(a, ref)
+    s.set_field("tmp", a)
+    s.ComputePsi("tmp", "psiS")
+    assert np.array_equal(s.get_field("psiS"), ref)
+    ref = a.copy()
+    o.truncate(ref)
+    s.Truncate("tmp")
+    assert np.array_equal(s.get_field("tmp"), ref)
+    w, psi = _turbulence(o, 2)
+    u, v = o.field(), o.field()
+    o.compute_velocity(u, v, psi, trans=True)
+    s.set_field("psiS", psi)
+    s.ComputeVelocity("u", "v", "psiS", trans=True)
+    for slot, r in (("u", u), ("v", v)):
+        e = rel_l2(s.rview(s.get_field(slot))[:, : o.nxG], o.rview(r)[:, : o.nxG])
+        assert e < 1e-14 * np.log2(o.nxG * o.nyG), slot
+    s.close()
+
+
+def test_compute_rhs_full_size():
+    oracle.set_threads(8)
+    N = 1024
+    o, s = _pair(N)
+    w, psi = _turbulence(o, 1, k0=10.0)
+    u, v, dW = o.field(), o.field(), o.field()
+    wo = w.copy()
+    o.compute_rhs(1e-4, u, v, wo, psi, dW)
+    s.set_field("w", w)
+    s.set_field("psi", psi)
+    s.ComputeRHS(1e-4)
+    e = rel_l2(s.cview(s.get_field("dW")), o.cview(dW))
+    print(f"  ComputeRHS N={N} rel_l2(dW)={e:.3e}", flush=True)
+    assert e < 1e-13
+    assert rel_l2(s.cview(s.get_field("w")), o.cview(wo)) < 1e-14
+    s.close()
+
+
+def test_integrate_one_step_literal_512():
+    """The literal IntegrateOneStep (per-routine kernels, 18 transforms per
+    step) at 512^2 (Np = 768) over a few CFL steps."""
+    oracle.set_threads(8)
+    N, steps = 512, 4
+    o, s, st = _run_pair(N, steps, 1e-4, fused=False)
+    e = rel_l2(s.cview(s.get_field("w")), o.cview(st["w"]))
+    print(f"  literal N={N} steps={steps} rel_l2(w)={e:.3e}", flush=True)
+    assert e < 1e-12
+    assert rel_l2(s.cview(s.get_field("psi")), o.cview(st["psi"])) < 1e-12
+    s.close()

@@ -79,3 +79,63 @@ def test_xmf_text():
     assert "               REST_00000003.h5:/FlowData/Omega" in lines
     assert "               GRID.h5:/Domain_00001/y" in lines
     assert lines[-1] == "</Xdmf>"
+
+
+# ---- libs2d_h5 (the writers behind the Fortran IO_m): the same files
+
+
+def _h5c():
+    import ctypes as C
+    from spectral2d_amd import build
+    if not os.path.exists(build.LIB_H5):
+        pytest.skip("libs2d_h5.so not built")
+    L = C.CDLL(build.LIB_H5)
+    dp = C.POINTER(C.c_double)
+    L.s2d_h5_write_rest.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, dp, dp, dp, dp]
+    L.s2d_h5_write_grid.argtypes = [C.c_char_p, C.c_int, C.c_int]
+    L.s2d_h5_write_xmf.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int]
+    return L, dp
+
+
+def test_c_rest_writer_matches_python_reader(tmp_path):
+    """Padded real views (rISize, nyG): only the first nxG of each row land
+    in the file, which the Python reader returns unchanged."""
+    L, dp = _h5c()
+    nxG, nyG, rI = 24, 18, 26
+    f = _fields(nxG, nyG, seed=3)
+    padded = {}
+    for k, v in f.items():
+        a = np.full((nyG, rI), np.nan)
+        a[:, :nxG] = v
+        padded[k] = np.ascontiguousarray(a)
+    p = tmp_path / "REST_00000004.h5"
+    rc = L.s2d_h5_write_rest(os.fsencode(str(p)), nxG, nyG, rI, 77, 1.25, 3e-4,
+                             *[padded[k].ctypes.data_as(dp) for k in ("Omega", "Psi", "u", "v")])
+    assert rc == 0
+    r = h5.read_rest_file(str(p))
+    assert (r["nx"], r["ny"], r["nadv"], r["time"], r["nu"]) == (nxG, nyG, 77, 1.25, 3e-4)
+    for k, v in f.items():
+        assert np.array_equal(r[k], v)
+    # invalid sizes are refused, not written
+    assert L.s2d_h5_write_rest(os.fsencode(str(p)), nxG, nyG, nxG - 1, 0, 0.0, 0.0,
+                               *[padded[k].ctypes.data_as(dp) for k in ("Omega", "Psi", "u",
+                                                                        "v")]) == 1
+
+
+def test_c_grid_and_xmf_match_python(tmp_path):
+    L, _ = _h5c()
+    pc, pp = tmp_path / "GRID_c.h5", tmp_path / "GRID_py.h5"
+    assert L.s2d_h5_write_grid(os.fsencode(str(pc)), 8, 6) == 0
+    h5.write_grid_file(str(pp), 8, 6)
+    H = h5.lib()
+    for name in ("Domain_00001/x", "Domain_00001/y"):
+        a, b = [H.H5Fopen(os.fsencode(str(q)), h5.H5F_ACC_RDONLY, h5.H5P_DEFAULT)
+                for q in (pc, pp)]
+        xa, xb = h5._read_dataset(a, name), h5._read_dataset(b, name)
+        H.H5Fclose(a)
+        H.H5Fclose(b)
+        assert np.array_equal(xa, xb)
+    px = tmp_path / "REST_00000003.xmf"
+    assert L.s2d_h5_write_xmf(os.fsencode(str(px)), 1536, 1536, 3) == 0
+    assert px.read_text() == h5.xmf_text(1536, 1536, 3)
