@@ -173,6 +173,18 @@ int s2d_rk3_steps(s2d_ctx *ctx, int nsteps, double nu, double dt, double cfl);
 int s2d_cfl_dt(s2d_ctx *ctx, double cfl, double *dt);
 /* FP64 device reductions of the current state. */
 int s2d_diagnostics(s2d_ctx *ctx, s2d_diag *out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the reductions of s2d_diagnostics on a caller's field, with
+ * no allocation of any kind. w: device pointer to a field of the context's
+ * grid and precision in the reference layout (cISize * nyG complex);
+ * out: device pointer to 5 doubles, receiving tke, enstrophy, palinstrophy,
+ * max |w_hat| and the count of non-finite values. Enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) without waiting. Its partial sums
+ * use one fixed device array per GPU (nyG <= 6144, i.e. N <= 4096 under the
+ * 3/2 rule; larger grids fail with S2D_ERR_UNSUPPORTED): calls that overlap
+ * on one GPU race. */
+int s2d_diagnostics_noalloc(s2d_ctx *ctx, void *stream, const void *w, double *out);
 /* ComputeSpectrum, src/Analysis.F90:267-460, on the device in FP64: shell
  * energies ek[nbins] of the current state (shell b: floor(|k| + 1/2) = b,
  * weights 1/2 at kx = 0 and kx = Np/2), the padded-spectrum mode counts

@@ -42,7 +42,8 @@ EXPORTS = (
     "s2d_transform_c2r", "s2d_dudx", "s2d_dudy", "s2d_truncate",
     "s2d_compute_psi", "s2d_compute_velocity", "s2d_compute_vorticity",
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
-    "s2d_cfl_dt", "s2d_diagnostics", "s2d_set_time", "s2d_get_time", "s2d_profile_begin",
+    "s2d_cfl_dt", "s2d_diagnostics", "s2d_diagnostics_noalloc", "s2d_set_time",
+    "s2d_get_time", "s2d_profile_begin",
     "s2d_profile_every", "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
     "s2d_spectrum_bins", "s2d_spectrum", "s2d_spectrum_uv", "s2d_ensemble_spectrum",
     "s2d_checkpoint_write", "s2d_checkpoint_read", "s2d_restrict", "s2d_axpby",
@@ -150,6 +151,7 @@ def lib():
     L.s2d_checkpoint_write.argtypes = [vp, C.c_char_p]
     L.s2d_checkpoint_read.argtypes = [vp, C.c_char_p]
     L.s2d_spectrum.argtypes = [vp, C.c_double, dp, ip, dp, dp, dp]
+    L.s2d_diagnostics_noalloc.argtypes = [vp, vp, vp, vp]
     L.s2d_spectrum_uv.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, ip, dp, dp, dp]
     L.s2d_ensemble_spectrum.argtypes = [vp, C.c_int, C.c_double, dp, ip, dp, dp, dp]
     L.s2d_ensemble_last_error.argtypes = [vp]

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1031,6 +1032,8 @@ void drop_closure_graphs(s2d_ctx *c)
 namespace {
 int create_ctx(int nx, int ny, int dealias, int precision, int nruns, hipStream_t stream,
                s2d_ctx **out);
+// the device address of s2d_diagnostics_noalloc's scratch (s2d_noalloc.inc)
+int noalloc_setup(s2d_ctx *c);
 
 // The seeded decaying-turbulence recipe (kept modes 0 < |k| <= kMax,
 // E(k) ~ k^4 exp(-(k/k0)^2), |u_k|^2 = E/(pi k), phase 2 pi U(seed, kx, ky),
@@ -1187,6 +1190,7 @@ int create_ctx(int nx, int ny, int dealias, int precision, int nruns, hipStream_
         return bail(S2D_ERR_HIP);
     }
     c->max_lds = (int)prop.sharedMemPerBlock;
+    if (noalloc_setup(c) != S2D_OK) return bail(S2D_ERR_HIP);
     {
         const char *e = std::getenv("S2D_GENERIC_KERNELS");
         c->wave_kernels = !(e && e[0] == '1');
@@ -1940,3 +1944,4 @@ int s2d_profile_end(s2d_ctx *c, double *ms, long long *count)
 
 #include "s2d_checkpoint.inc"
 #include "s2d_ensemble.inc"
+#include "s2d_noalloc.inc"
