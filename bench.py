@@ -57,6 +57,9 @@ def alg_bytes_per_launch(N: int, s: int):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
+# the autotuner's choices (s2d_tune / s2d_ensemble_tune), JSON lines keyed by
+# device, precision and grid: an entry is used as is, a missing one measured
+TUNE_CACHE = os.path.join(ROOT, "spectral2d_amd", "data", "tune_mi355x.jsonl")
 PROFILE_EVERY = 20  # kernel timing: one step in twenty of the timed region
 # kernel-name patterns of the three passes per workload
 KERNEL_TAGS = {"c1": ("k2_passA<", "k2_passB<", "k2_passC<"),
@@ -167,6 +170,7 @@ def run_ensemble(steps: int, warmup: int, cfl: float):
     e = Ensemble(sizes, FP32)
     e.set_viscosity(nu)
     e.set_turbulence(seeds, k0, 0.5)
+    sched, tuned_us = e.tune(cfl=cfl, cache=TUNE_CACHE)  # plan-time, state restored
     if warmup > 0:
         e.steps(warmup, cfl=cfl, graph=True)
     t0 = time.perf_counter()
@@ -210,7 +214,33 @@ def run_ensemble(steps: int, warmup: int, cfl: float):
                                  "gbs": round(per[k] / (avg_ms[k] * 1e-3) / 1e9, 1),
                                  "traffic": traffic.get(tags[k])} for k in range(3)},
         "sim_time_range": [min(x["time"] for x in d), max(x["time"] for x in d)],
+        "schedule": {0: "unified", 1: "per-class", 2: "per-class tuned"}[sched],
+        "err_vs_fp64": ensemble_err_vs_fp64(sizes, k0, nu, seeds, cfl),
     }
+
+
+def ensemble_err_vs_fp64(sizes, k0, nu, seeds, cfl: float, steps: int = 100):
+    """Relative L2 distance of omega between FP32 and FP64 runs of one member
+    per size (the first drawn of each), from the same initial condition over
+    the same fixed steps (dt: the FP64 run's initial CFL step). An FP32 run
+    alone is bitwise the ensemble member (tests/test_gpu_ensemble.py)."""
+    import numpy as np
+    from spectral2d_amd.spectral import FP32, FP64, Spectral2D
+    out = {}
+    for n in (128, 256, 512):
+        m = int(np.flatnonzero(sizes == n)[0])
+        w = {}
+        for prec in (FP64, FP32):
+            s = Spectral2D(n, precision=prec)
+            s.SetTurbulence(int(seeds[m]), float(k0[m]), 0.5)
+            if prec == FP64:
+                dt = s.ComputeTimeStep(cfl)
+            s.rk3_steps(steps, float(nu[m]), dt, 0.0)
+            w[prec] = s.get_field("w")
+            s.close()
+        out[f"{n}^2 member {m}"] = float(np.linalg.norm(w[FP32] - w[FP64]) /
+                                          np.linalg.norm(w[FP64]))
+    return {"rel_l2_omega": out, "steps": steps, "dt": "fixed, the FP64 run's initial CFL step"}
 
 
 def run_c1_fp32(N: int, steps: int, warmup: int, nu: float, cfl: float):
@@ -219,6 +249,7 @@ def run_c1_fp32(N: int, steps: int, warmup: int, nu: float, cfl: float):
     sim = Spectral2D(N, precision=FP32)
     sim.SetTurbulence(seed=1, k0=10.0, ke=0.5)
     sim.set_time(0.0, 0)
+    sched = sim.tune(nu, cfl=cfl, cache=TUNE_CACHE)
     sim.rk3_steps(warmup, nu, 0.0, cfl)
     t0 = time.perf_counter()
     sim.rk3_steps(steps, nu, 0.0, cfl)
@@ -228,8 +259,36 @@ def run_c1_fp32(N: int, steps: int, warmup: int, nu: float, cfl: float):
     return {"metric": "Mpts·RK3-steps/s (FP32 context)", "value": round(N * N / 1e6 * steps / el, 3),
             "unit": UNIT, "steps": steps, "ms_per_step": round(el / steps * 1e3, 5), "dtype": "f32",
             "alg_gbs": round(bps * steps / el / 1e9, 1),
+            "schedule": schedule_summary(sched),
+            "err_vs_fp64": c1_err_vs_fp64(N, nu, cfl),
             "config": {"workload": f"decaying 2D turbulence {N}^2 (padded {3 * N // 2}^2), FP32, "
                                    f"CFL {cfl}, nu {nu}"}}
+
+
+def c1_err_vs_fp64(N: int, nu: float, cfl: float, steps: int = 200):
+    """Relative L2 distance of omega between the FP32 and FP64 GPU paths
+    from the same initial condition over the same fixed steps (dt: the FP64
+    run's initial CFL step)."""
+    import numpy as np
+    from spectral2d_amd.spectral import FP32, FP64, Spectral2D
+    w = {}
+    for prec in (FP64, FP32):
+        s = Spectral2D(N, precision=prec)
+        s.SetTurbulence(seed=1, k0=10.0, ke=0.5)
+        if prec == FP64:
+            dt = s.ComputeTimeStep(cfl)
+        s.rk3_steps(steps, nu, dt, 0.0)
+        w[prec] = s.get_field("w")
+        s.close()
+    return {"rel_l2_omega": float(np.linalg.norm(w[FP32] - w[FP64]) / np.linalg.norm(w[FP64])),
+            "steps": steps, "dt": "fixed, the FP64 run's initial CFL step"}
+
+
+def schedule_summary(r: dict) -> dict:
+    """The tuner's choice as the bench line reports it."""
+    return {"passes": [r["pass_a"], r["pass_b"], r["pass_c"]], "step_graph": bool(r["step_graph"]),
+            "from_cache": bool(r["from_cache"]), "tuned_us_per_step": round(r["step_us"], 2),
+            "default_us_per_step": round(r["default_us"], 2)}
 
 
 def run_host_exchange(N: int, steps: int, nu: float, cfl: float):
@@ -285,9 +344,32 @@ def run_closure(members: int):
     }
 
 
-def cpu_baseline(N: int, nu: float, cfl: float, budget_s: float):
+def host_cpu() -> dict:
+    """The host's CPU model, physical core count and the CPUs this process
+    may use (/proc/cpuinfo, the affinity mask)."""
+    model, cores = "unknown", set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "physical_cores": len(cores) or None, "available_cpus": avail}
+
+
+def _oracle_rate(N: int, nu: float, cfl: float, threads: int, budget_s: float, max_steps: int):
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
     oracle.set_threads(threads)
     o = oracle.Oracle(N)
     st = o.new_state()
@@ -297,19 +379,36 @@ def cpu_baseline(N: int, nu: float, cfl: float, budget_s: float):
     t0 = time.perf_counter()
     o.step(st, nu, dt)
     one = time.perf_counter() - t0
-    n = int(max(2, min(50, budget_s / max(one, 1e-3))))
+    n = int(max(2, min(max_steps, budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(n):
         dt = o.cfl_dt(st["psi"], cfl)
         o.step(st, nu, dt)
     el = time.perf_counter() - t0
+    return (N * N / 1e6) * n / el, n
+
+
+def cpu_baseline(N: int, nu: float, cfl: float, budget_s: float, gpu_value: float):
+    """The C oracle (a restatement of the reference algorithm with its own
+    radix-2/3 FFT and OpenMP loops, one process) on the host: with the CPU
+    share of one GPU (16 threads on these boxes, at most the CPUs available)
+    and with one thread (P = 1)."""
+    cpu = host_cpu()
+    threads = max(1, min(16, cpu["available_cpus"] or 1))
+    value, n = _oracle_rate(N, nu, cfl, threads, budget_s, 50)
+    p1, n1 = _oracle_rate(N, nu, cfl, 1, min(budget_s, 5.0), 10)
     return {
-        "value": (N * N / 1e6) * n / el,
+        "value": value,
         "unit": UNIT,
         "cores": threads,
         "kind": "port",
         "sample": (f"{n} RK3 steps (CFL dt each step) of the {N}^2 FP64 workload on "
                    f"{threads} host threads, C oracle with its own radix-2/3 FFT"),
+        "cpu_model": cpu["model"],
+        "host_physical_cores": cpu["physical_cores"],
+        "host_available_cpus": cpu["available_cpus"],
+        "p1": {"value": p1, "sample": f"{n1} steps on 1 thread"},
+        "gpu_speedup": round(gpu_value / value, 1),
     }
 
 
@@ -375,14 +474,17 @@ def main(argv=None):
     sim = Spectral2D(N, precision=prec)
     sim.SetTurbulence(seed=1, k0=10.0, ke=0.5)
     sim.set_time(0.0, 0)
+    # plan-time schedule choice (FFTW_MEASURE's part): cached, or measured
+    # from this state, which the tuner restores
+    sched = sim.tune(args.nu, cfl=args.cfl, cache=TUNE_CACHE)
     if args.warmup > 0:
         sim.rk3_steps(args.warmup, args.nu, 0.0, args.cfl)
 
-    # timed region: K steps; the fused launches of one step in every
-    # PROFILE_EVERY are bracketed by HIP events on the library's stream
-    # (each event pair adds a dispatch gap of a few microseconds, so timing
-    # every launch would slow the measured steps); rk3_steps synchronises
-    # before returning
+    # timed region: K steps, replays of one captured step (hipGraph); one
+    # step in every PROFILE_EVERY runs eagerly with each fused launch
+    # bracketed by HIP events on the library's stream (each event pair adds
+    # a dispatch gap of a few microseconds, so timing every launch would
+    # slow the measured steps); rk3_steps synchronises before returning
     sim.profile_begin(9 * ((args.steps + PROFILE_EVERY - 1) // PROFILE_EVERY), PROFILE_EVERY)
     t0 = time.perf_counter()
     sim.rk3_steps(args.steps, args.nu, 0.0, args.cfl)
@@ -424,6 +526,7 @@ def main(argv=None):
                                f"{args.precision.upper()} drop-in, CFL {args.cfl}, nu {args.nu}",
                    "N": N, "padded": 3 * N // 2, "nu": args.nu, "cfl": args.cfl,
                    "sim_time": diag["time"]},
+        "schedule": schedule_summary(sched),
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -439,7 +542,7 @@ def main(argv=None):
         if args.closure_members > 0:
             out["closure"] = run_closure(args.closure_members)
     if args.cpu_budget > 0:
-        out["cpu_baseline"] = cpu_baseline(N, args.nu, args.cfl, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(N, args.nu, args.cfl, args.cpu_budget, value)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -267,6 +267,40 @@ int s2d_set_turbulence(s2d_ctx *ctx, unsigned long long seed, double k0, double 
  * (1-based i), then TransformR2C: SetTaylorGreen, src/SetIC.F90:70-101. */
 int s2d_set_taylor_green(s2d_ctx *ctx);
 
+/* ---- schedule of the fused step and its autotuner ----
+ * No reference counterpart beyond FFTW's plan-time measurement
+ * (FFTW_MEASURE in SetupPlans, src/Spectral.F90:340-349). A schedule picks,
+ * per pass of the fused step, one of the kernel variants compiled for the
+ * context's padded size (index 0 is the default; the single-run sizes
+ * Np = 768, 1536, 3072 carry alternatives: the other twiddle batching, the
+ * other pass B row pairing and, in FP32, 24 elements per lane for passes A
+ * and C), and whether multi-step calls replay one captured step (hipGraph).
+ * Every variant computes the same mathematics; results differ at rounding
+ * level only. */
+typedef struct s2d_schedule {
+    int pass_a, pass_b, pass_c; /* variant index of each pass */
+    int step_graph;             /* 1: multi-step calls replay a captured step */
+    double step_us;             /* s2d_tune: measured microseconds per step */
+    double default_us;          /* s2d_tune: the default schedule's, eager */
+    int candidates, rejected;   /* s2d_tune: schedules timed / failing the check */
+    int from_cache;             /* s2d_tune: 1 when read from the cache */
+} s2d_schedule;
+#define S2D_TUNE_MEASURE 1      /* measure even when the cache has an entry */
+#define S2D_TUNE_CACHE_ONLY 2   /* never measure: the cache's entry or the default */
+/* n[3]: the variants compiled for each pass at this context's size. */
+int s2d_schedule_variants(const s2d_ctx *ctx, int *n);
+int s2d_get_schedule(const s2d_ctx *ctx, s2d_schedule *s);
+int s2d_set_schedule(s2d_ctx *ctx, const s2d_schedule *s);
+/* Time every schedule on the context's own buffers from its current state
+ * (steps with nu, dt, cfl as in s2d_rk3_steps), reject any whose step
+ * differs from the default's by more than rounding (relative L2 of the new
+ * state 1e-12 in FP64, 1e-5 in FP32), set the fastest and restore the state
+ * (time and step count included). cache: a JSON-lines file (NULL or "": none)
+ * keyed by device, precision and grid; without S2D_TUNE_MEASURE an entry in
+ * it is used as is. out (may be NULL) receives the choice. */
+int s2d_tune(s2d_ctx *ctx, double nu, double dt, double cfl, const char *cache, int flags,
+             s2d_schedule *out);
+
 /* ---- kernel timing (benchmarks) ----
  * Between s2d_profile_begin and s2d_profile_end every launch of the fused
  * stage kernels is bracketed by HIP events recorded on the context's stream
@@ -326,7 +360,19 @@ int s2d_ensemble_diagnostics(s2d_ensemble *ens, s2d_diag *out);
  * compute the same bits. */
 #define S2D_ENS_UNIFIED 0
 #define S2D_ENS_PER_CLASS 1
+#define S2D_ENS_PER_CLASS_TUNED 2 /* s2d_ensemble_tune only: per class, each class's variants */
 int s2d_ensemble_set_schedule(s2d_ensemble *ens, int schedule);
+/* The schedule by measurement (the s2d_tune of an ensemble, FFTW_MEASURE's
+ * part): the unified launches, the per-class launches with the default
+ * variants, and the per-class launches with each class's variants tuned by
+ * s2d_tune's search, each checked against the unified step (bitwise for the
+ * first two, to rounding for the third) and timed by graph replay from the
+ * current state, which is restored; the fastest is set. cache and flags as
+ * in s2d_tune (the key holds the classes' sizes and run counts). *schedule
+ * receives S2D_ENS_UNIFIED, S2D_ENS_PER_CLASS or S2D_ENS_PER_CLASS_TUNED,
+ * *step_us the measured microseconds per step (either may be NULL). */
+int s2d_ensemble_tune(s2d_ensemble *ens, double dt, double cfl, const char *cache, int flags,
+                      int *schedule, double *step_us);
 /* Kernel timing of the ensemble launches (eager steps only), as
  * s2d_profile_begin / s2d_profile_end. */
 int s2d_ensemble_profile_begin(s2d_ensemble *ens, int max_launches);

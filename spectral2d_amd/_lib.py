@@ -59,11 +59,16 @@ EXPORTS = (
     "s2d_ensemble_steps", "s2d_ensemble_diagnostics", "s2d_ensemble_set_schedule",
     "s2d_ensemble_profile_begin",
     "s2d_ensemble_profile_end",
+    "s2d_schedule_variants", "s2d_get_schedule", "s2d_set_schedule", "s2d_tune",
+    "s2d_ensemble_tune",
 )
 
 ENS_GRAPH = 1
 ENS_UNIFIED = 0
 ENS_PER_CLASS = 1
+ENS_PER_CLASS_TUNED = 2
+TUNE_MEASURE = 1
+TUNE_CACHE_ONLY = 2
 
 
 class Grid(C.Structure):
@@ -83,6 +88,14 @@ class Diag(C.Structure):
     _fields_ = [
         ("tke", C.c_double), ("enstrophy", C.c_double), ("palinstrophy", C.c_double),
         ("max_abs_w", C.c_double), ("time", C.c_double), ("nadv", C.c_longlong),
+    ]
+
+
+class Schedule(C.Structure):
+    _fields_ = [
+        ("pass_a", C.c_int), ("pass_b", C.c_int), ("pass_c", C.c_int), ("step_graph", C.c_int),
+        ("step_us", C.c_double), ("default_us", C.c_double),
+        ("candidates", C.c_int), ("rejected", C.c_int), ("from_cache", C.c_int),
     ]
 
 
@@ -192,6 +205,12 @@ def lib():
     L.s2d_set_closure_guard.argtypes = [vp, C.c_int]
     L.s2d_closure_steps.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_int]
+    L.s2d_schedule_variants.argtypes = [vp, ip]
+    L.s2d_get_schedule.argtypes = [vp, P(Schedule)]
+    L.s2d_set_schedule.argtypes = [vp, P(Schedule)]
+    L.s2d_tune.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_int,
+                           P(Schedule)]
+    L.s2d_ensemble_tune.argtypes = [vp, C.c_double, C.c_double, C.c_char_p, C.c_int, ip, dp]
     for name in EXPORTS:
         if name == "s2d_cnn_param_count":
             L.s2d_cnn_param_count.restype = C.c_longlong

@@ -34,14 +34,30 @@
 
 namespace s2d {
 
-template <typename T, int NP>
+// Schedule variants of the pass kernels, a bit mask V (0: the default
+// schedule below). A tuner (s2d_tune) times the compiled variants of each
+// pass and picks one per context; every variant computes the same
+// mathematics (the twiddle bits change rounding at the ulp level, the
+// others only the order of work).
+//   VAR_TW_A / _B / _C  the pass's twiddles the other way: all loaded in one
+//                       batch (TW_ALL) where the default forms the final
+//                       pass's as powers, or the reverse
+//   VAR_PAIR            pass B with one lane group per two rows where the
+//                       default has one per row, or the reverse
+//   VAR_E24             24 elements per lane (radix-8 early passes and a
+//                       radix-24 final pass) instead of 12
+constexpr int VAR_TW_A = 1, VAR_TW_B = 2, VAR_TW_C = 4, VAR_PAIR = 8, VAR_E24 = 16;
+
+constexpr int tw_flip(int tw) { return tw == TW_ALL ? (TW_EARLY | TW_FINAL_POW) : TW_ALL; }
+
+template <typename T, int NP, int V = 0>
 struct WaveCfg {
-    static constexpr int E = 12;
+    static constexpr int E = (V & VAR_E24) ? 24 : 12;
     static constexpr int TG = NP / E;
     // lines (columns / rows) per block of passes A and B, LPB * TG threads
     static constexpr int LPB_A = TG >= 256 ? (sizeof(T) == 4 ? 2 : 1) : 256 / TG;
     // pass B: two rows per lane group from Np = 1536 up (passB_pair)
-    static constexpr bool PAIR_B = NP >= 1536;
+    static constexpr bool PAIR_B = (NP >= 1536) != ((V & VAR_PAIR) != 0);
     static constexpr int LPB_B = PAIR_B ? 2 : TG >= 128 && sizeof(T) == 8 ? 1 : TG >= 256 ? 2 : 256 / TG;
     static constexpr int THR_B = PAIR_B ? LPB_B * TG / 2 : LPB_B * TG; // lanes per block
     // pass C: column groups per half, two halves per block
@@ -51,16 +67,18 @@ struct WaveCfg {
     // register budgets, waves per SIMD, of passes A and C (FP64 pass C: 256
     // VGPRs hold a transform's data, its twiddles and the loads issued ahead
     // of it without spills; FP64 pass A fits three waves in 168 by loading
-    // only the early-pass twiddles in one batch)
-    static constexpr int WPE_A = sizeof(T) == 8 ? 3 : 4;
-    static constexpr int WPE_C = sizeof(T) == 8 ? 2 : TG <= 64 ? 5 : 4;
+    // only the early-pass twiddles in one batch; 24 elements per lane hold
+    // twice the data: two waves)
+    static constexpr int WPE_A = E == 24 ? 2 : sizeof(T) == 8 ? 3 : 4;
+    static constexpr int WPE_C = E == 24 ? 2 : sizeof(T) == 8 ? 2 : TG <= 64 ? 5 : 4;
     // pass C: the compact-state operands loaded ahead of the transform, and
     // the twiddles in one batch; FP64 and the ensemble's FP32 sizes form the
     // final pass's as powers of W^t (FP64: 168 VGPRs, three waves per SIMD,
     // 7 % less time at Np = 1536; FP32 Np <= 768: five waves, 7 % less
     // time for the mixed ensemble)
-    static constexpr int TW_PRE_C =
+    static constexpr int TW_PRE_C0 =
         sizeof(T) == 8 || TG <= 64 ? (TW_EARLY | TW_FINAL_POW) : TW_ALL;
+    static constexpr int TW_PRE_C = (V & VAR_TW_C) ? tw_flip(TW_PRE_C0) : TW_PRE_C0;
     static constexpr bool OPS_EARLY_C = true;
     // pass A: FP64 loads its early-pass twiddles in one batch (1.3 % less
     // time at Np = 1536 than all in one batch at two waves) and forms the
@@ -70,16 +88,20 @@ struct WaveCfg {
     // early-pass ones per pass and forms the final pass's as powers (4 %
     // less time for the mixed ensemble than all in one batch, then 2.6 %
     // less than loading the final ones too)
-    static constexpr int TW_PRE_A =
+    static constexpr int TW_PRE_A0 =
         sizeof(T) == 8 ? (TW_EARLY | TW_FINAL_POW) : TG >= 128 ? TW_ALL : TW_FINAL_POW;
+    static constexpr int TW_PRE_A = (V & VAR_TW_A) ? tw_flip(TW_PRE_A0) : TW_PRE_A0;
     // pass B: the lane's twiddles stay in registers for all of its row
     // transforms; FP64 has room next to the data (256 VGPRs) for the
     // early-pass ones only (measured: 9 % less time at Np = 1536 than
     // loading every twiddle per pass), and forms the final pass's as powers
     // of W^t (5 % less time than loading them)
-    static constexpr int TW_PRE_B = sizeof(T) == 4 ? TW_ALL : (TW_EARLY | TW_FINAL_POW);
+    static constexpr int TW_PRE_B0 = sizeof(T) == 4 ? TW_ALL : (TW_EARLY | TW_FINAL_POW);
+    static constexpr int TW_PRE_B = (V & VAR_TW_B) ? tw_flip(TW_PRE_B0) : TW_PRE_B0;
+    // (one row per lane group: every twiddle in one batch by default)
+    static constexpr int TW_PRE_B1 = (V & VAR_TW_B) ? tw_flip(TW_ALL) : TW_ALL;
     // pass B holds w and u + i v; a multi-wave group needs more registers
-    static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64) ? 2 : 3;
+    static constexpr int WPE_B = (sizeof(T) == 8 || TG > 64 || E == 24) ? 2 : 3;
     static_assert(E * TG == NP && TG >= 2 && TG <= 256 && (TG & (TG - 1)) == 0,
                   "unsupported group FFT size");
 };
@@ -88,9 +110,9 @@ struct WaveCfg {
 // line, or per two rows in passB_pair).
 // The LDS slices are NP + 16 elements apart, so that the transposed reads
 // (LPB slices at one index) spread evenly over the banks.
-template <typename T, int NP, int LPB>
+template <typename T, int NP, int LPB, int V = 0>
 struct TileCfg {
-    static constexpr int TG = WaveCfg<T, NP>::TG;
+    static constexpr int TG = WaveCfg<T, NP, V>::TG;
     static constexpr int BLK = LPB * TG;
     static constexpr int SL = NP + 16;
     static constexpr int LDS = LPB * SL;
@@ -226,7 +248,7 @@ __host__ __device__ constexpr int passA_tiles(int K, int LPB)
     return passA_tail(K, LPB) ? (K - 1) / LPB : (K + LPB - 1) / LPB;
 }
 
-template <typename T, int NP, int LPB>
+template <typename T, int NP, int LPB, int V = 0>
 __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Wsrc,
                                            cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
@@ -235,8 +257,8 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
                                            int reset_max, const Batch &bt)
 {
 #pragma clang fp contract(off)
-    using C = WaveCfg<T, NP>;
-    using Tl = TileCfg<T, NP, LPB>;
+    using C = WaveCfg<T, NP, V>;
+    using Tl = TileCfg<T, NP, LPB, V>;
     constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL;
     const int g = threadIdx.x / TG, t = threadIdx.x % TG;
     const int tiles = passA_tiles(d.K, LPB);
@@ -341,16 +363,16 @@ __host__ __device__ constexpr int passA_blocks(int K, int nruns)
     return 3 * passA_tiles(K, LPB) * nruns;
 }
 
-template <typename T, int NP>
-__global__ __launch_bounds__((TileCfg<T, NP, WaveCfg<T, NP>::LPB_A>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_A))) void
+template <typename T, int NP, int V = 0>
+__global__ __launch_bounds__((TileCfg<T, NP, WaveCfg<T, NP, V>::LPB_A, V>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP, V>::WPE_A))) void
 k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
          cpx<T> *__restrict__ Gw, FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc,
          int reset_max, Batch bt)
 {
-    constexpr int LPB = WaveCfg<T, NP>::LPB_A;
-    __shared__ cpx<T> lds[TileCfg<T, NP, LPB>::LDS];
-    passA_body<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc, reset_max,
+    constexpr int LPB = WaveCfg<T, NP, V>::LPB_A;
+    __shared__ cpx<T> lds[TileCfg<T, NP, LPB, V>::LDS];
+    passA_body<T, NP, LPB, V>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc, reset_max,
                            bt);
 }
 
@@ -390,7 +412,7 @@ __device__ __forceinline__ cpx<T> herm_load(const cpx<T> *__restrict__ row, int 
 // single-row tiles keep more waves in flight where blocks are many and
 // short (the mixed ensemble); the pairs win where the transform is long
 // (Np >= 1536).
-template <typename T, int NP, int LPB>
+template <typename T, int NP, int LPB, int V = 0>
 __device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Gu,
                                            const cpx<T> *__restrict__ Gv,
@@ -401,8 +423,8 @@ __device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
                                            const Batch &bt)
 {
 #pragma clang fp contract(off)
-    using C = WaveCfg<T, NP>;
-    using Tl = TileCfg<T, NP, LPB>;
+    using C = WaveCfg<T, NP, V>;
+    using Tl = TileCfg<T, NP, LPB, V>;
     constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL; // BLK lanes
     constexpr int tiles = NP / LPB;
     static_assert(NP % LPB == 0, "row tiles must divide the rows");
@@ -434,7 +456,7 @@ __device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
         v[s] = herm_load<T, NP, TG, s>(rw, t);
     });
     if constexpr (sizeof(T) == 4) load_uv(z);
-    GroupTw<T, E, TG> gt; // the row's three transforms share the lane's twiddles
+    GroupTw<T, E, TG, C::TW_PRE_B1> gt; // the row's three transforms share the lane's twiddles
     gt.load(tw, t);
     group_fft<T, E, TG, 1>(v, slice, t, gt);
     T wr[E];
@@ -495,7 +517,7 @@ __device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
         atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
 }
 
-template <typename T, int NP, int LPB>
+template <typename T, int NP, int LPB, int V = 0>
 __device__ __forceinline__ void passB_pair(int blk, int nblk, cpx<T> *lds,
                                            const cpx<T> *__restrict__ Gu,
                                            const cpx<T> *__restrict__ Gv,
@@ -506,8 +528,8 @@ __device__ __forceinline__ void passB_pair(int blk, int nblk, cpx<T> *lds,
                                            const Batch &bt)
 {
 #pragma clang fp contract(off)
-    using C = WaveCfg<T, NP>;
-    using Tl = TileCfg<T, NP, LPB>;
+    using C = WaveCfg<T, NP, V>;
+    using Tl = TileCfg<T, NP, LPB, V>;
     constexpr int E = C::E, TG = C::TG, THR = Tl::BLK / 2, SL = Tl::SL;
     constexpr int tiles = NP / LPB;
     static_assert(NP % LPB == 0 && LPB % 2 == 0, "row tiles: even, dividing the rows");
@@ -626,21 +648,21 @@ __host__ __device__ constexpr int passB_blocks(int nruns)
     return NP / LPB * nruns;
 }
 
-template <typename T, int NP>
-__global__ __launch_bounds__((WaveCfg<T, NP>::THR_B))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_B))) void
+template <typename T, int NP, int V = 0>
+__global__ __launch_bounds__((WaveCfg<T, NP, V>::THR_B))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP, V>::WPE_B))) void
 k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
          const cpx<T> *__restrict__ Gw, cpx<T> *__restrict__ Huw, cpx<T> *__restrict__ Hvw,
          FusedDims d, const cpx<T> *__restrict__ tw, DevScalars *sc, int forward, int track_max,
          Batch bt)
 {
-    constexpr int LPB = WaveCfg<T, NP>::LPB_B;
-    __shared__ cpx<T> lds[TileCfg<T, NP, LPB>::LDS];
-    if constexpr (WaveCfg<T, NP>::PAIR_B)
-        passB_pair<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc,
+    constexpr int LPB = WaveCfg<T, NP, V>::LPB_B;
+    __shared__ cpx<T> lds[TileCfg<T, NP, LPB, V>::LDS];
+    if constexpr (WaveCfg<T, NP, V>::PAIR_B)
+        passB_pair<T, NP, LPB, V>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc,
                                forward, track_max, bt);
     else
-        passB_single<T, NP, LPB>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc,
+        passB_single<T, NP, LPB, V>(blockIdx.x, gridDim.x, lds, Gu, Gv, Gw, Huw, Hvw, d, tw, sc,
                                  forward, track_max, bt);
 }
 
@@ -649,9 +671,9 @@ k2_passB(const cpx<T> *__restrict__ Gu, const cpx<T> *__restrict__ Gv,
 // The compact-state operands of half HALF's rows (s with s % 2 == HALF) of
 // one column: the stage and step-start values and the forcing, loaded
 // before the column's transform so that their latency overlaps it.
-template <typename T, int NP, int HALF>
+template <typename T, int NP, int HALF, int V = 0>
 struct PassCOperands {
-    static constexpr int E = WaveCfg<T, NP>::E, TG = WaveCfg<T, NP>::TG;
+    static constexpr int E = WaveCfg<T, NP, V>::E, TG = WaveCfg<T, NP, V>::TG;
     cpx<T> ws[E / 2], w0[E / 2], fv[E / 2];
     int c[E / 2];
     bool row[E / 2];
@@ -699,15 +721,15 @@ __device__ __forceinline__ cpx<T> rk3_update(int stage, T dt, cpx<T> w0v, cpx<T>
 // The RHS, truncation and stage update of half HALF's rows of column kx;
 // P / Q are the uw / vw column spectra (one of them in registers, the other
 // read from the partner half's LDS slice).
-template <typename T, int NP, int HALF>
-__device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP>::E],
+template <typename T, int NP, int HALF, int V = 0>
+__device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP, V>::E],
                                              const cpx<T> *other, int t, int kx, bool active,
-                                             const PassCOperands<T, NP, HALF> &op, bool forced,
+                                             const PassCOperands<T, NP, HALF, V> &op, bool forced,
                                              cpx<T> *Wout, size_t col, const StageArgs &a,
                                              double dtd, double nu)
 {
 #pragma clang fp contract(off)
-    using C = WaveCfg<T, NP>;
+    using C = WaveCfg<T, NP, V>;
     constexpr int E = C::E, TG = C::TG, H = NP / 3;
     const T dt = (T)dtd;
     const T qx = (T)kx;
@@ -753,14 +775,14 @@ constexpr bool passC_tail_on()
 }
 
 // One half of a pass C block: half 0 transforms uw, half 1 vw.
-template <typename T, int NP, int HALF, bool TAIL>
+template <typename T, int NP, int HALF, bool TAIL, int V = 0>
 __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
                                            const cpx<T> *__restrict__ Hsrc, const cpx<T> *Wn,
                                            const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
                                            const cpx<T> *__restrict__ tw, DevScalars *sc,
                                            const StageArgs &a, const Batch &bt)
 {
-    using C = WaveCfg<T, NP>;
+    using C = WaveCfg<T, NP, V>;
     constexpr int E = C::E, TG = C::TG;
     const int g = lane / TG, t = lane % TG;
     cpx<T> *mine = lds_all + (HALF * C::GPB + g) * C::SLICE;
@@ -775,7 +797,7 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
     const size_t col = run * bt.sW + (size_t)kx * d.KP;
     const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
-    PassCOperands<T, NP, HALF> op;
+    PassCOperands<T, NP, HALF, V> op;
     if constexpr (C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
     GroupTw<T, E, TG, C::TW_PRE_C> gt;
     gt.load(tw, t);
@@ -789,7 +811,7 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     for (int s = 0; s < E; ++s)
         if ((s & 1) != HALF) mine[LdsPad<T>::idx_sum(it, TG * s)] = v[s];
     __syncthreads();
-    passC_finish<T, NP, HALF>(v, other, t, kx, active, op, Fc != nullptr, Wout, col, a, dtd, nu);
+    passC_finish<T, NP, HALF, V>(v, other, t, kx, active, op, Fc != nullptr, Wout, col, a, dtd, nu);
     if (HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0) {
         sc[run].dt = dtd;
         sc[run].time = sc[run].time + dtd;
@@ -804,14 +826,14 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
 // block's reduction), the RHS and stage update as passC_finish's, and the
 // column's other rows are written as zeros. Every thread of the block
 // takes part.
-template <typename T, int NP>
+template <typename T, int NP, int V = 0>
 __device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T> *Huw,
                                            const cpx<T> *Wn, const cpx<T> *Wst, cpx<T> *Wout,
                                            const FusedDims &d, DevScalars *sc,
                                            const StageArgs &a, const Batch &bt)
 {
 #pragma clang fp contract(off)
-    constexpr int NT = 2 * WaveCfg<T, NP>::BLK, NW = NT / 64;
+    constexpr int NT = 2 * WaveCfg<T, NP, V>::BLK, NW = NT / 64;
     const int kx = d.K - 1;
     const cpx<T> *h = Huw + run * bt.sH + (size_t)kx * NP;
     cpx<T> p = mk<T>((T)0, (T)0);
@@ -841,7 +863,7 @@ __device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T
     for (int c = 1 + threadIdx.x; c <= d.N; c += NT) Wout[col + c] = mk<T>((T)0, (T)0);
 }
 
-template <typename T, int NP, bool TAIL = false>
+template <typename T, int NP, bool TAIL = false, int V = 0>
 __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
                                            const cpx<T> *__restrict__ Huw,
                                            const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
@@ -849,32 +871,32 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
                                            const cpx<T> *__restrict__ tw, DevScalars *sc,
                                            const StageArgs &a, const Batch &bt)
 {
-    using C = WaveCfg<T, NP>;
+    using C = WaveCfg<T, NP, V>;
     const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK;
     // wave-uniform; both halves pass the same barriers
     if (half == 0)
-        passC_half<T, NP, 0, TAIL>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);
+        passC_half<T, NP, 0, TAIL, V>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);
     else
-        passC_half<T, NP, 1, TAIL>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
+        passC_half<T, NP, 1, TAIL, V>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
     if constexpr (TAIL) {
         // the block holding column 0 of a run also writes the run's column K - 1
         const int Kt = d.K - 1, j0 = blk * C::GPB;
         const int tr = (j0 + Kt - 1) / Kt; // first run whose column 0 is at or after j0
         if (tr < bt.nruns && tr * Kt < j0 + C::GPB) // block-uniform
-            passC_tail<T, NP>(tr, lds_all, Huw, Wn, Wst, Wout, d, sc, a, bt);
+            passC_tail<T, NP, V>(tr, lds_all, Huw, Wn, Wst, Wout, d, sc, a, bt);
     }
 }
 
-template <typename T, int NP>
-__global__ __launch_bounds__((2 * WaveCfg<T, NP>::BLK))
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP>::WPE_C))) void
+template <typename T, int NP, int V = 0>
+__global__ __launch_bounds__((2 * WaveCfg<T, NP, V>::BLK))
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, NP, V>::WPE_C))) void
 k2_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
          const cpx<T> *Wst, cpx<T> *Wout, FusedDims d, const cpx<T> *__restrict__ tw,
          const int *__restrict__ trunc_i1, DevScalars *sc, StageArgs a, Batch bt)
 {
-    using C = WaveCfg<T, NP>;
+    using C = WaveCfg<T, NP, V>;
     __shared__ cpx<T> lds_all[2 * C::GPB * C::SLICE];
-    passC_body<T, NP, passC_tail_on<NP>()>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw,
+    passC_body<T, NP, passC_tail_on<NP>(), V>(blockIdx.x, lds_all, Huw, Hvw, Wn, Wst, Wout, d, tw,
                                           sc, a, bt);
 }
 

@@ -638,27 +638,27 @@ Batch batch_of(const s2d_ctx *c)
     return b;
 }
 
-template <typename T, int NP>
+template <typename T, int NP, int V = 0>
 int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
 {
-    constexpr int LPB = WaveCfg<T, NP>::LPB_A;
+    constexpr int LPB = WaveCfg<T, NP, V>::LPB_A;
     FusedDims d = fdims(c);
     int blocks = passA_blocks<T, NP, LPB>(d.K, c->nruns);
     blocks = (blocks + 7) / 8 * 8; // a multiple of 8 for the XCD-major order
-    launch(c, k2_passA<T, NP>, dim3(blocks), dim3(TileCfg<T, NP, LPB>::BLK), 0, src, (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2],
+    launch(c, k2_passA<T, NP, V>, dim3(blocks), dim3(TileCfg<T, NP, LPB, V>::BLK), 0, src, (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2],
                        d, (const cpx<T> *)c->col.tw, c->d_sc, reset, batch_of(c));
     S2D_LAUNCHED(c, "k2_passA");
     return S2D_OK;
 }
 
-template <typename T, int NP>
+template <typename T, int NP, int V = 0>
 int launch2_passB(s2d_ctx *c, int forward, int track_max)
 {
-    constexpr int LPB = WaveCfg<T, NP>::LPB_B;
+    constexpr int LPB = WaveCfg<T, NP, V>::LPB_B;
     FusedDims d = fdims(c);
     int blocks = passB_blocks<T, NP, LPB>(c->nruns);
     blocks = (blocks + 7) / 8 * 8;
-    launch(c, k2_passB<T, NP>, dim3(blocks), dim3(WaveCfg<T, NP>::THR_B), 0,
+    launch(c, k2_passB<T, NP, V>, dim3(blocks), dim3(WaveCfg<T, NP, V>::THR_B), 0,
            (const cpx<T> *)c->G[0], (const cpx<T> *)c->G[1],
                        (const cpx<T> *)c->G[2], (cpx<T> *)c->H[0], (cpx<T> *)c->H[1], d,
                        (const cpx<T> *)c->rowf.tw, c->d_sc, forward, track_max, batch_of(c));
@@ -666,20 +666,62 @@ int launch2_passB(s2d_ctx *c, int forward, int track_max)
     return S2D_OK;
 }
 
-template <typename T, int NP>
+template <typename T, int NP, int V = 0>
 int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &a)
 {
-    using Cf = WaveCfg<T, NP>;
+    using Cf = WaveCfg<T, NP, V>;
     FusedDims d = fdims(c);
     // the transformed columns (with passC_tail, each run's column K - 1 is
     // written by the block holding its column 0)
     const int cols = passC_tail_on<NP>() ? d.K - 1 : d.K;
     const int blocks = (cols * c->nruns + Cf::GPB - 1) / Cf::GPB;
-    launch(c, k2_passC<T, NP>, dim3(blocks), dim3(2 * Cf::BLK), 0, (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
+    launch(c, k2_passC<T, NP, V>, dim3(blocks), dim3(2 * Cf::BLK), 0, (const cpx<T> *)c->H[0], (const cpx<T> *)c->H[1], (const cpx<T> *)c->Wn,
                        Wst, Wout, d, (const cpx<T> *)c->col.tw, c->d_trunc, c->d_sc, a,
                        batch_of(c));
     S2D_LAUNCHED(c, "k2_passC");
     return S2D_OK;
+}
+
+// The compiled schedule variants (VAR_* bit masks, kernels_fused_v2.hpp) of
+// pass P (0 = A, 1 = B, 2 = C) at padded size NP; index 0 is the default.
+// The single-run sizes 768, 1536 and 3072 carry alternatives: the other
+// twiddle mode for every pass, the other pass B row pairing, and (FP32) 24
+// elements per lane for passes A and C. (24 per lane spills FP64 passes and
+// every pass B.) The ensemble's smaller sizes have the default only.
+template <typename T, int NP>
+constexpr bool tune_size()
+{
+    return NP == 768 || NP == 1536 || NP == 3072;
+}
+
+template <typename T, int NP, int P>
+constexpr int n_variants()
+{
+    if constexpr (!tune_size<T, NP>()) return 1;
+    else if constexpr (P == 1) return 3;
+    else return sizeof(T) == 4 ? 3 : 2;
+}
+
+template <typename T, int NP, int P, int I>
+constexpr int variant_bits()
+{
+    static_assert(I < n_variants<T, NP, P>(), "no such variant");
+    if constexpr (I == 0) return 0;
+    else if constexpr (P == 0) return I == 1 ? VAR_TW_A : VAR_E24;
+    else if constexpr (P == 1) return I == 1 ? VAR_TW_B : VAR_PAIR;
+    else return I == 1 ? VAR_TW_C : VAR_E24;
+}
+
+// f(std::integral_constant<int, bits>) for variant idx of pass P.
+template <typename T, int NP, int P, int I = 0, typename F>
+int with_variant(int idx, F &&f)
+{
+    if constexpr (I < n_variants<T, NP, P>()) {
+        if (idx == I) return f(std::integral_constant<int, variant_bits<T, NP, P, I>()>{});
+        return with_variant<T, NP, P, I + 1>(idx, f);
+    } else {
+        return S2D_ERR_INVALID_ARG;
+    }
 }
 
 template <typename T, int NP>
@@ -701,15 +743,21 @@ int fused_steps_wave(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
             a.stage = stage;
             {
                 ProfScope ps(c, S2D_KERNEL_PASS_A);
-                S2D_TRY((launch2_passA<T, NP>(c, src, stage == 1)));
+                S2D_TRY((with_variant<T, NP, 0>(c->var[0], [&](auto v) {
+                    return launch2_passA<T, NP, decltype(v)::value>(c, src, stage == 1);
+                })));
             }
             {
                 ProfScope ps(c, S2D_KERNEL_PASS_B);
-                S2D_TRY((launch2_passB<T, NP>(c, 1, stage == 1)));
+                S2D_TRY((with_variant<T, NP, 1>(c->var[1], [&](auto v) {
+                    return launch2_passB<T, NP, decltype(v)::value>(c, 1, stage == 1);
+                })));
             }
             {
                 ProfScope ps(c, S2D_KERNEL_PASS_C);
-                S2D_TRY((launch2_passC<T, NP>(c, src, dst, a)));
+                S2D_TRY((with_variant<T, NP, 2>(c->var[2], [&](auto v) {
+                    return launch2_passC<T, NP, decltype(v)::value>(c, src, dst, a);
+                })));
             }
         }
     }
@@ -1026,6 +1074,62 @@ void drop_closure_graphs(s2d_ctx *c)
     }
 }
 
+void drop_step_graph(s2d_ctx *c)
+{
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+    c->step_graph = nullptr;
+}
+
+// Multi-step calls of a single run with the compiled wave kernels replay
+// one captured step (hipGraph): the step's 9 launches then go out without
+// host round trips between them. The graph is re-captured when an argument
+// baked into it changes (nu, dt, cfl, the forcing buffer, the schedule).
+// Inside a timing session the sampled steps run eagerly with their events.
+template <typename T>
+int fused_steps_graphed(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
+{
+    const void *forcing = c->forcing_on ? c->forcing : nullptr;
+    const bool fresh = c->step_graph && c->sg_nu == nu && c->sg_dt == dt && c->sg_cfl == cfl &&
+                       c->sg_forcing == forcing && c->sg_var[0] == c->var[0] &&
+                       c->sg_var[1] == c->var[1] && c->sg_var[2] == c->var[2];
+    if (!fresh) {
+        drop_step_graph(c);
+        const bool prof = c->prof_on;
+        c->prof_on = false; // no timing events inside the graph
+        hipGraph_t g = nullptr;
+        hipError_t eb = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        int r = eb == hipSuccess ? fused_steps<T>(c, 1, nu, dt, cfl) : S2D_OK;
+        const hipError_t ec = eb == hipSuccess ? hipStreamEndCapture(c->stream, &g) : eb;
+        c->prof_on = prof;
+        if (r != S2D_OK || ec != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            return r != S2D_OK ? r
+                               : fail(c, S2D_ERR_HIP, "capture of the fused step failed: %s",
+                                      hipGetErrorString(ec));
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->step_graph = nullptr;
+            return fail(c, S2D_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        }
+        c->sg_nu = nu;
+        c->sg_dt = dt;
+        c->sg_cfl = cfl;
+        c->sg_forcing = forcing;
+        for (int k = 0; k < 3; ++k) c->sg_var[k] = c->var[k];
+    }
+    for (int st = 0; st < nsteps; ++st) {
+        if (c->prof_on && c->prof_step % c->prof_every == 0) {
+            S2D_TRY(fused_steps<T>(c, 1, nu, dt, cfl)); // a sampled step, with its events
+        } else {
+            if (c->prof_on) ++c->prof_step;
+            S2D_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
+        }
+    }
+    return S2D_OK;
+}
+
 
 } // namespace
 
@@ -1330,6 +1434,7 @@ int s2d_destroy(s2d_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     drop_closure_graphs(c);
+    drop_step_graph(c);
     for (int f = 0; f < S2D_NFIELDS; ++f) (void)hipFree(c->field[f]);
     void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->d_nu, c->forcing, c->col.tw,
                   c->rowh.tw, c->rowf.tw, c->Wn, c->Ws, c->G[0], c->G[1], c->G[2], c->H[0],
@@ -1561,7 +1666,12 @@ int s2d_rk3_steps(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
     }
     S2D_TRY(ensure_compact(c));
     if (nsteps == 0) return sync(c);
-    S2D_TRY(S2D_DISPATCH_T(c, fused_steps, nsteps, nu, dt, cfl));
+    const bool graphed = c->step_graph_on && c->nruns == 1 &&
+                         (c->precision == S2D_FP64 ? wave_np<double>(c) : wave_np<float>(c)) != 0;
+    if (graphed)
+        S2D_TRY(S2D_DISPATCH_T(c, fused_steps_graphed, nsteps, nu, dt, cfl));
+    else
+        S2D_TRY(S2D_DISPATCH_T(c, fused_steps, nsteps, nu, dt, cfl));
     c->ref_stale = true;
     {
         const size_t n = (size_t)c->K * c->KP;
@@ -1945,3 +2055,4 @@ int s2d_profile_end(s2d_ctx *c, double *ms, long long *count)
 #include "s2d_checkpoint.inc"
 #include "s2d_ensemble.inc"
 #include "s2d_noalloc.inc"
+#include "s2d_tune.inc"

@@ -81,6 +81,16 @@ struct s2d_ctx {
     // once this reaches its period); large: no forcing computed yet
     int cl_age = 1 << 30;
 
+    // schedule of the fused step (s2d_set_schedule / s2d_tune): the variant
+    // index of passes A, B, C among those compiled for the padded size, and
+    // whether multi-step calls replay a captured step (hipGraph)
+    int var[3] = {0, 0, 0};
+    bool step_graph_on = true;
+    hipGraphExec_t step_graph = nullptr;
+    double sg_nu = 0.0, sg_dt = 0.0, sg_cfl = 0.0; // the captured step's arguments
+    const void *sg_forcing = nullptr;              // its forcing buffer (or null)
+    int sg_var[3] = {0, 0, 0};
+
     // fused compact state
     bool fused = false;
     bool wave_kernels = true; // compiled wave-FFT kernels for the hot sizes

@@ -106,6 +106,19 @@ class Ensemble:
         self._c(L.lib().s2d_ensemble_set_schedule(
             self._h, L.ENS_PER_CLASS if per_class else L.ENS_UNIFIED))
 
+    def tune(self, dt: float = 0.0, cfl: float = 0.5, cache: str | None = None,
+             measure: bool = False, cache_only: bool = False):
+        """s2d_ensemble_tune: the launch schedule (and the classes' kernel
+        variants) by measurement from the current state, which is restored.
+        Returns (schedule, microseconds per step)."""
+        sched = C.c_int()
+        us = C.c_double()
+        flags = (L.TUNE_MEASURE if measure else 0) | (L.TUNE_CACHE_ONLY if cache_only else 0)
+        self._c(L.lib().s2d_ensemble_tune(self._h, float(dt), float(cfl),
+                                          cache.encode() if cache else None, flags,
+                                          C.byref(sched), C.byref(us)))
+        return sched.value, us.value
+
     def spectrum(self, run: int, nu: float = 0.0):
         """Shell spectrum of one run: (ek, cnt, tke, eps, enstrophy_dissipation)."""
         nb = L.lib().s2d_spectrum_bins(C.byref(self.grids[run]))

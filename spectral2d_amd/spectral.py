@@ -228,6 +228,33 @@ class Spectral2D:
                                         C.byref(t), C.byref(e), C.byref(z)))
         return ek, cnt, t.value, e.value, z.value
 
+    # ---- schedule of the fused step and its autotuner (s2d_tune)
+    def schedule_variants(self):
+        """Variants compiled for passes A, B, C at this size."""
+        n = (C.c_int * 3)()
+        self._c(L.lib().s2d_schedule_variants(self._h, n))
+        return tuple(n)
+
+    def get_schedule(self) -> dict:
+        s = L.Schedule()
+        self._c(L.lib().s2d_get_schedule(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.Schedule._fields_}
+
+    def set_schedule(self, pass_a=0, pass_b=0, pass_c=0, step_graph=True):
+        s = L.Schedule(pass_a, pass_b, pass_c, int(bool(step_graph)))
+        self._c(L.lib().s2d_set_schedule(self._h, C.byref(s)))
+
+    def tune(self, nu: float, dt: float = 0.0, cfl: float = 0.5, cache: str | None = None,
+             measure: bool = False, cache_only: bool = False) -> dict:
+        """Pick the fastest schedule by measurement from the current state
+        (FFTW_MEASURE's part; the state is restored), caching the choice in a
+        JSON-lines file."""
+        s = L.Schedule()
+        flags = (L.TUNE_MEASURE if measure else 0) | (L.TUNE_CACHE_ONLY if cache_only else 0)
+        self._c(L.lib().s2d_tune(self._h, float(nu), float(dt), float(cfl),
+                                 cache.encode() if cache else None, flags, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.Schedule._fields_}
+
     def restrict_to(self, f, dst: "Spectral2D", df, kcut: float):
         """dst's slot df = the modes |k| <= kcut of this context's slot f
         (sharp spectral cutoff, sampled on dst's grid)."""
