@@ -118,11 +118,18 @@ def eval_args(**kw):
     return argparse.Namespace(**base)
 
 
-def evaluate(a, net, log=log):
+def evaluate(a, net, log=log, extra=None):
+    """Held-out members: DNS to t0, then from its filtered state a plain
+    256^2 run, a closure run with `net` and, for each (name, network) of
+    `extra`, a run with that network (same guard and refresh period), all
+    to t1 while the DNS continues; errors of each kind's ensemble-mean
+    enstrophy spectrum against the filtered DNS."""
     from .closure import ClosedRun, FilteredSampler, Timer, advance_to, enstrophy_spectrum
     from .closure import spectrum_error
+    extra = dict(extra or {})
     kmax = a.nc // 2
     Z = {"dns": [], "plain": [], "closure": []}
+    Z.update({k: [] for k in extra})
     t_dns, t_clo, t_pln = Timer(), Timer(), Timer()
     steps = {"dns": 0, "plain": 0, "closure": 0}
     unstable = {}
@@ -130,9 +137,9 @@ def evaluate(a, net, log=log):
         s = make_dns(a.nf, a.eval_seed0 + m, a.k0, a.nu)
         advance_to(s, a.t0, a.nu, a.cfl)
         runs = {}
-        for kind in ("plain", "closure"):
-            r = ClosedRun(a.nc, net if kind == "closure" else None, every=a.closure_every,
-                          guard=bool(a.guard))
+        nets = {"plain": None, "closure": net, **extra}
+        for kind in ("plain", "closure", *extra):
+            r = ClosedRun(a.nc, nets[kind], every=a.closure_every, guard=bool(a.guard))
             s.restrict_to("w", r.sim, "w", kmax)
             r.sim.ComputePsi("w", "psi")
             r.sim.set_time(s.get_time()[0], 0)
@@ -148,8 +155,8 @@ def evaluate(a, net, log=log):
         s.close()
         for kind, r in runs.items():
             try:
-                with (t_clo if kind == "closure" else t_pln):
-                    steps[kind] += r.advance_to(a.t1, a.nu, a.cfl, log=log)
+                with (t_clo if kind == "closure" else t_pln if kind == "plain" else Timer()):
+                    steps[kind] = steps.get(kind, 0) + r.advance_to(a.t1, a.nu, a.cfl, log=log)
                 Z[kind].append(enstrophy_spectrum(r.sim))
             except RuntimeError as ex:
                 log(f"  {kind} run of member {m} unstable: {ex}")
@@ -162,7 +169,14 @@ def evaluate(a, net, log=log):
     if not Z["closure"]:
         return {"members": a.eval_runs, "unstable": unstable, "steps": steps}
     zd, zp, zc = (np.mean(Z[k], 0) for k in ("dns", "plain", "closure"))
+    for k in extra:
+        if len(Z[k]) == len(Z["dns"]):
+            out_extra = spectrum_error(np.mean(Z[k], 0), zd, kmax)
+        else:  # a member of this kind went unstable
+            out_extra = float("inf")
+        extra[k] = out_extra
     return {
+        **{f"err_{k}": v for k, v in extra.items()},
         "unstable": unstable,
         "members": a.eval_runs,
         "t0": a.t0, "t1": a.t1,

@@ -88,3 +88,44 @@ def test_gelu_closed_forms():
     for v in (-2.0, -0.3, 0.4, 2.5):
         fd = (cnn.gelu(np.array([v + 1e-6])) - cnn.gelu(np.array([v - 1e-6])))[0] / 2e-6
         assert abs(fd - cnn.gelu_grad(np.array([v]))[0]) < 1e-8
+
+
+def test_every_layer_matches_torch_circular_convolution():
+    """All six layers (97 % of the flops are the four 64 -> 64 ones) against
+    an independent route: torch's cross-correlation on the CPU in FP64 with
+    circular padding, the weights laid out from the documented parameter
+    order (layer 0 [tap, ci, co], layers 1-4 [tap, co, ci], layer 5
+    [tap, ci], tap t = 3 dy + dx). A transposed weight or a wrong tap offset
+    in any layer fails it."""
+    torch = pytest.importorskip("torch")
+    import torch.nn.functional as F
+
+    x, _ = _data(B=2, H=8, W=12, seed=7)
+    p = cnn.init_params(11)
+    P = cnn.unpack(p)
+    rng = np.random.default_rng(2)
+    for k in P:  # nonzero biases, so that a dropped bias shows
+        if k.startswith("b"):
+            P[k] = rng.standard_normal(P[k].shape)
+    p = cnn.pack(P)
+    y, c = cnn.forward(p, x, in_scale=(0.8, 1.1), round=False)
+
+    def conv(a_nhwc, w_oikk, b):
+        a = torch.from_numpy(np.ascontiguousarray(a_nhwc.transpose(0, 3, 1, 2)))
+        a = F.pad(a, (1, 1, 1, 1), mode="circular")
+        z = F.conv2d(a, torch.from_numpy(np.ascontiguousarray(w_oikk)),
+                     torch.from_numpy(np.asarray(b, np.float64)))
+        return z.numpy().transpose(0, 2, 3, 1)
+
+    def w_from(taps_co_ci):  # (9, co, ci) -> (co, ci, 3, 3)
+        return taps_co_ci.reshape(3, 3, *taps_co_ci.shape[1:]).transpose(2, 3, 0, 1)
+
+    z = conv(c["x0"], w_from(P["W0"].transpose(0, 2, 1)), P["b0"])
+    assert np.allclose(z, c["z0"], rtol=1e-12, atol=1e-12)
+    for l in range(1, cnn.LAYERS_MID + 1):
+        z = conv(c[f"a{l - 1}"], w_from(P[f"W{l}"]), P[f"b{l}"])
+        assert np.allclose(z, c[f"z{l}"], rtol=1e-12, atol=1e-12), l
+    z = conv(c[f"a{cnn.LAYERS_MID}"], w_from(P["W5"][:, None, :]), P["b5"])
+    assert np.allclose(z[..., 0], y, rtol=1e-12, atol=1e-12)
+    # the input scaling of the two channels
+    assert np.allclose(c["x0"][..., 0], 0.8 * x[:, 0]) and np.allclose(c["x0"][..., 1], 1.1 * x[:, 1])
