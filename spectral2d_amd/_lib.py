@@ -67,6 +67,7 @@ ENS_GRAPH = 1
 ENS_UNIFIED = 0
 ENS_PER_CLASS = 1
 ENS_PER_CLASS_TUNED = 2
+ENS_UNIFIED_FUSED_A = 3
 TUNE_MEASURE = 1
 TUNE_CACHE_ONLY = 2
 

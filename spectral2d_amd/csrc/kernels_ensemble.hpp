@@ -82,19 +82,28 @@ __device__ __forceinline__ int ens_class(const EnsArgs<T> &ea, int p, int &b)
     case 768: { constexpr int NP = 768; CALL; } break;                   \
     }
 
-template <typename T>
+// V = 0: a block per (field, column tile); V = VAR_FUSE_A: a block per
+// column tile for the three fields (passA_fused_body; the class block counts
+// of EnsArgs follow the variant).
+template <typename T, int V = 0>
 __global__ __launch_bounds__(EnsCfg<T>::BLK_A)
-__attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192>::WPE_A))) void
+__attribute__((amdgpu_waves_per_eu(WaveCfg<T, 192, V>::WPE_A))) void
 k3_ens_passA(EnsArgs<T> ea, int stage)
 {
     __shared__ cpx<T> lds[EnsCfg<T>::LDS_A];
     int b = blockIdx.x;
     const EnsClass<T> &c = ea.c[ens_class(ea, 0, b)];
     const cpx<T> *src = stage == 1 ? c.Wn : c.Ws;
-    S2D_ENS_SIZES(T, c.np,
-                  (passA_body<T, NP, EnsCfg<T>::lpb_a(NP)>(b, c.nblk[0], lds, src, c.G0, c.G1,
-                                                          c.G2, c.d, c.tw_col, c.sc, stage == 1,
-                                                          c.bt)))
+    if constexpr ((V & VAR_FUSE_A) != 0)
+        S2D_ENS_SIZES(T, c.np,
+                      (passA_fused_body<T, NP, EnsCfg<T>::lpb_a(NP), V>(
+                          b, c.nblk[0], lds, src, c.G0, c.G1, c.G2, c.d, c.tw_col, c.sc,
+                          stage == 1, c.bt)))
+    else
+        S2D_ENS_SIZES(T, c.np,
+                      (passA_body<T, NP, EnsCfg<T>::lpb_a(NP)>(b, c.nblk[0], lds, src, c.G0,
+                                                              c.G1, c.G2, c.d, c.tw_col, c.sc,
+                                                              stage == 1, c.bt)))
 }
 
 template <typename T>

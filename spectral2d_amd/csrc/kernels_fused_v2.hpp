@@ -46,7 +46,11 @@ namespace s2d {
 //                       default has one per row, or the reverse
 //   VAR_E24             24 elements per lane (radix-8 early passes and a
 //                       radix-24 final pass) instead of 12
-constexpr int VAR_TW_A = 1, VAR_TW_B = 2, VAR_TW_C = 4, VAR_PAIR = 8, VAR_E24 = 16;
+//   VAR_FUSE_A          pass A with one block per column tile for all three
+//                       fields (the column loaded and psi formed once, three
+//                       transforms in turn) instead of one block per field
+constexpr int VAR_TW_A = 1, VAR_TW_B = 2, VAR_TW_C = 4, VAR_PAIR = 8, VAR_E24 = 16,
+              VAR_FUSE_A = 32;
 
 constexpr int tw_flip(int tw) { return tw == TW_ALL ? (TW_EARLY | TW_FINAL_POW) : TW_ALL; }
 
@@ -69,7 +73,8 @@ struct WaveCfg {
     // of it without spills; FP64 pass A fits three waves in 168 by loading
     // only the early-pass twiddles in one batch; 24 elements per lane hold
     // twice the data: two waves)
-    static constexpr int WPE_A = E == 24 ? 2 : sizeof(T) == 8 ? 3 : 4;
+    static constexpr int WPE_A = (V & VAR_FUSE_A) ? (sizeof(T) == 8 ? 2 : 3)
+                                 : E == 24 ? 2 : sizeof(T) == 8 ? 3 : 4;
     static constexpr int WPE_C = E == 24 ? 2 : sizeof(T) == 8 ? 2 : TG <= 64 ? 5 : 4;
     // pass C: the compact-state operands loaded ahead of the transform, and
     // the twiddles in one batch; FP64 and the ensemble's FP32 sizes form the
@@ -356,11 +361,111 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     }
 }
 
-// Blocks of pass A for nruns runs.
-template <typename T, int NP, int LPB>
+// Pass A with the three fields of a column tile in one block (VAR_FUSE_A):
+// the column's loads, kept flags and psi = w / |k|^2 are formed once, then
+// u_hat, v_hat and w_hat are transformed and stored in turn through the same
+// LDS tile (a barrier between the fields). Per field the arithmetic is
+// passA_body's, so the G fields are bitwise the same.
+template <typename T, int NP, int LPB, int V>
+__device__ __forceinline__ void passA_fused_body(int blk, int nblk, cpx<T> *lds,
+                                                 const cpx<T> *__restrict__ Wsrc,
+                                                 cpx<T> *__restrict__ Gu, cpx<T> *__restrict__ Gv,
+                                                 cpx<T> *__restrict__ Gw, const FusedDims &d,
+                                                 const cpx<T> *__restrict__ tw, DevScalars *sc,
+                                                 int reset_max, const Batch &bt)
+{
+#pragma clang fp contract(off)
+    using C = WaveCfg<T, NP, V>;
+    using Tl = TileCfg<T, NP, LPB, V>;
+    constexpr int E = C::E, TG = C::TG, BLK = Tl::BLK, SL = Tl::SL;
+    const int g = threadIdx.x / TG, t = threadIdx.x % TG;
+    const int tiles = passA_tiles(d.K, LPB);
+    const int job = xcd_major(blk, nblk);
+    if (job >= tiles * bt.nruns) return; // the whole block
+    const int run = job / tiles, kx0 = (job - run * tiles) * LPB;
+    const int kx = kx0 + g;
+    const bool active = kx < d.K;
+    if (reset_max && kx0 == 0 && threadIdx.x == 0) {
+        sc[run].vmax_bits = 0ull;
+        sc[run].bad = 0u;
+    }
+    const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
+    const T qx = (T)kx;
+    cpx<T> x[E];
+    bool kept[E];
+    static_for<0, E>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        bool kp;
+        const int c = compact_row_of<NP, TG, s>(t + TG * s, &kp);
+        kept[s] = kp && active;
+        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) x[s] = src[c];
+    });
+    GroupTw<T, E, TG, C::TW_PRE_A> gt;
+    gt.load(tw, t);
+    cpx<T> ps[E], v[E];
+    static_for<0, E>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        ps[s] = mk<T>((T)0, (T)0);
+        v[s] = mk<T>((T)0, (T)0);
+        const int kyi = signed_k<NP, TG, s>(t + TG * s);
+        const T qy = (T)kyi;
+        if constexpr (pad_region<NP, TG, s>() != PAD_ZERO) {
+            const cpx<T> w = sel(kept[s], x[s], ps[s]);
+            const T dd = (kx == 0) ? qy * qy : (kyi == 0) ? qx * qx : qx * qx + qy * qy;
+            const T inv = recip(dd == (T)0 ? (T)1 : dd); // one reciprocal per mode
+            ps[s] = sel(kx == 0 && kyi == 0, ps[s], mk<T>(w.re * inv, w.im * inv));
+            v[s] = sel(kept[s], x[s], v[s]); // w_hat (x and kept end here)
+        }
+    });
+    cpx<T> *slice = lds + g * SL;
+    const int it = LdsPad<T>::idx(t);
+    const int gi = threadIdx.x % LPB, yb = threadIdx.x / LPB;
+    // fields in the order w, u, v: w_hat first frees the column's registers
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        const int f = i == 0 ? 2 : i - 1;
+        if (i > 0) {
+            static_for<0, E>([&](auto sI) {
+                constexpr int s = decltype(sI)::value;
+                const T qy = (T)signed_k<NP, TG, s>(t + TG * s);
+                const T a = (f == 0) ? qy : -qx;
+                v[s] = mk<T>(-(a * ps[s].im), a * ps[s].re);
+            });
+            __syncthreads(); // the previous field's tile has been read
+        }
+        group_fft<T, E, TG, 1>(v, slice, t, gt);
+#pragma unroll
+        for (int s = 0; s < E; ++s) slice[LdsPad<T>::idx_sum(it, TG * s)] = v[s];
+        __syncthreads();
+        if (kx0 + gi < d.K) {
+            cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kx0 + gi;
+            const cpx<T> *sl = lds + gi * SL;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int y = yb + TG * m;
+                col[(size_t)y * d.KG] = sl[LdsPad<T>::idx(y)];
+            }
+        }
+        if (passA_tail(d.K, LPB) && kx0 + LPB == d.K - 1) { // block-uniform
+            const int kt = d.K - 1;
+            const cpx<T> w0 = Wsrc[run * bt.sW + (size_t)kt * d.KP];
+            const T qt = (T)kt;
+            const T inv = recip(qt * qt);
+            const cpx<T> pt = mk<T>(w0.re * inv, w0.im * inv);
+            const T a = (f == 0) ? (T)0 : -qt;
+            const cpx<T> val = f == 2 ? w0 : mk<T>(-(a * pt.im), a * pt.re);
+            cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kt;
+            for (int y = threadIdx.x; y < NP; y += BLK) col[(size_t)y * d.KG] = val;
+        }
+    }
+}
+
+// Blocks of pass A for nruns runs (V: the schedule variant; VAR_FUSE_A takes
+// the three fields of a tile in one block).
+template <typename T, int NP, int LPB, int V = 0>
 __host__ __device__ constexpr int passA_blocks(int K, int nruns)
 {
-    return 3 * passA_tiles(K, LPB) * nruns;
+    return ((V & VAR_FUSE_A) ? 1 : 3) * passA_tiles(K, LPB) * nruns;
 }
 
 template <typename T, int NP, int V = 0>
@@ -372,8 +477,12 @@ k2_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu, cpx<T> *__res
 {
     constexpr int LPB = WaveCfg<T, NP, V>::LPB_A;
     __shared__ cpx<T> lds[TileCfg<T, NP, LPB, V>::LDS];
-    passA_body<T, NP, LPB, V>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc, reset_max,
-                           bt);
+    if constexpr ((V & VAR_FUSE_A) != 0)
+        passA_fused_body<T, NP, LPB, V>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc,
+                                        reset_max, bt);
+    else
+        passA_body<T, NP, LPB, V>(blockIdx.x, gridDim.x, lds, Wsrc, Gu, Gv, Gw, d, tw, sc,
+                                  reset_max, bt);
 }
 
 // ------------------------------------------------------------- pass B

@@ -643,7 +643,7 @@ int launch2_passA(s2d_ctx *c, const cpx<T> *src, int reset)
 {
     constexpr int LPB = WaveCfg<T, NP, V>::LPB_A;
     FusedDims d = fdims(c);
-    int blocks = passA_blocks<T, NP, LPB>(d.K, c->nruns);
+    int blocks = passA_blocks<T, NP, LPB, V>(d.K, c->nruns);
     blocks = (blocks + 7) / 8 * 8; // a multiple of 8 for the XCD-major order
     launch(c, k2_passA<T, NP, V>, dim3(blocks), dim3(TileCfg<T, NP, LPB, V>::BLK), 0, src, (cpx<T> *)c->G[0], (cpx<T> *)c->G[1], (cpx<T> *)c->G[2],
                        d, (const cpx<T> *)c->col.tw, c->d_sc, reset, batch_of(c));
@@ -685,9 +685,10 @@ int launch2_passC(s2d_ctx *c, const cpx<T> *Wst, cpx<T> *Wout, const StageArgs &
 // The compiled schedule variants (VAR_* bit masks, kernels_fused_v2.hpp) of
 // pass P (0 = A, 1 = B, 2 = C) at padded size NP; index 0 is the default.
 // The single-run sizes 768, 1536 and 3072 carry alternatives: the other
-// twiddle mode for every pass, the other pass B row pairing, and (FP32) 24
-// elements per lane for passes A and C. (24 per lane spills FP64 passes and
-// every pass B.) The ensemble's smaller sizes have the default only.
+// twiddle mode for every pass, the other pass B row pairing, (FP32) 24
+// elements per lane for passes A and C, and pass A with the three fields of
+// a tile in one block. (24 per lane spills FP64 passes and every pass B.)
+// The ensemble's smaller sizes have the default only.
 template <typename T, int NP>
 constexpr bool tune_size()
 {
@@ -698,6 +699,7 @@ template <typename T, int NP, int P>
 constexpr int n_variants()
 {
     if constexpr (!tune_size<T, NP>()) return 1;
+    else if constexpr (P == 0) return sizeof(T) == 4 ? 4 : 3;
     else if constexpr (P == 1) return 3;
     else return sizeof(T) == 4 ? 3 : 2;
 }
@@ -707,7 +709,8 @@ constexpr int variant_bits()
 {
     static_assert(I < n_variants<T, NP, P>(), "no such variant");
     if constexpr (I == 0) return 0;
-    else if constexpr (P == 0) return I == 1 ? VAR_TW_A : VAR_E24;
+    else if constexpr (P == 0)
+        return I == 1 ? VAR_TW_A : (I == 2 && sizeof(T) == 4) ? VAR_E24 : VAR_FUSE_A;
     else if constexpr (P == 1) return I == 1 ? VAR_TW_B : VAR_PAIR;
     else return I == 1 ? VAR_TW_C : VAR_E24;
 }
