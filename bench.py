@@ -179,6 +179,8 @@ def run_ensemble(steps: int, warmup: int, cfl: float):
     # per-kernel durations: eager launches of the same step, events on the
     # ensemble's stream (a graph replay records no per-kernel events)
     kt = max(2, min(20, steps))
+    if sched == 4:  # one kernel per stage: the per-pass figures are the unified launches'
+        e.set_schedule()
     e.profile_begin(9 * kt)
     e.steps(kt, cfl=cfl)
     prof = e.profile_end()
@@ -214,7 +216,8 @@ def run_ensemble(steps: int, warmup: int, cfl: float):
                                  "gbs": round(per[k] / (avg_ms[k] * 1e-3) / 1e9, 1),
                                  "traffic": traffic.get(tags[k])} for k in range(3)},
         "sim_time_range": [min(x["time"] for x in d), max(x["time"] for x in d)],
-        "schedule": {0: "unified", 1: "per-class", 2: "per-class tuned"}[sched],
+        "schedule": {0: "unified", 1: "per-class", 2: "per-class tuned", 3: "unified, fused pass A",
+                     4: "stage queue"}[sched],
         "err_vs_fp64": ensemble_err_vs_fp64(sizes, k0, nu, seeds, cfl),
     }
 

@@ -358,22 +358,27 @@ int s2d_ensemble_diagnostics(s2d_ensemble *ens, s2d_diag *out);
  * kernel pass covering every size class; S2D_ENS_UNIFIED_FUSED_A, the same
  * with pass A's three fields of a column tile in one block; S2D_ENS_PER_CLASS,
  * one launch per pass and size class of the kernels compiled for that size
- * alone. All compute the same bits. */
+ * alone; S2D_ENS_STAGE_QUEUE, one launch per RK stage whose blocks take the
+ * stage's pass A, B and C items from a queue, a run's pass B items waiting
+ * for its pass A items and its pass C items for its pass B items. All compute
+ * the same bits. */
 #define S2D_ENS_UNIFIED 0
 #define S2D_ENS_PER_CLASS 1
 #define S2D_ENS_PER_CLASS_TUNED 2 /* s2d_ensemble_tune only: per class, each class's variants */
 #define S2D_ENS_UNIFIED_FUSED_A 3 /* unified, pass A with the three fields of a tile per block */
+#define S2D_ENS_STAGE_QUEUE 4     /* one kernel per RK stage over a work queue (per-run pass
+                                     dependencies instead of kernel boundaries) */
 int s2d_ensemble_set_schedule(s2d_ensemble *ens, int schedule);
 /* The schedule by measurement (the s2d_tune of an ensemble, FFTW_MEASURE's
- * part): the unified launches (with either pass A), the per-class launches
- * with the default variants, and the per-class launches with each class's
- * variants tuned by s2d_tune's search, each checked against the unified step
- * (bitwise, but to rounding for the last) and timed by graph replay from the
+ * part): the unified launches (with either pass A), the stage queue
+ * kernel, the per-class launches with the default variants, and the
+ * per-class launches with each class's variants tuned by s2d_tune's search,
+ * each checked against the unified step (bitwise, but to rounding for the
+ * last) and timed by graph replay from the
  * current state, which is restored; the fastest is set. cache and flags as
  * in s2d_tune (the key holds the classes' sizes and run counts). *schedule
- * receives S2D_ENS_UNIFIED, S2D_ENS_UNIFIED_FUSED_A, S2D_ENS_PER_CLASS or
- * S2D_ENS_PER_CLASS_TUNED, *step_us the measured microseconds per step
- * (either may be NULL). */
+ * receives the chosen S2D_ENS_* schedule code, *step_us the measured
+ * microseconds per step (either may be NULL). */
 int s2d_ensemble_tune(s2d_ensemble *ens, double dt, double cfl, const char *cache, int flags,
                       int *schedule, double *step_us);
 /* Kernel timing of the ensemble launches (eager steps only), as

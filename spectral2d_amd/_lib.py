@@ -68,6 +68,7 @@ ENS_UNIFIED = 0
 ENS_PER_CLASS = 1
 ENS_PER_CLASS_TUNED = 2
 ENS_UNIFIED_FUSED_A = 3
+ENS_STAGE_QUEUE = 4
 TUNE_MEASURE = 1
 TUNE_CACHE_ONLY = 2
 

@@ -978,10 +978,13 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
                                            const cpx<T> *__restrict__ Hvw, const cpx<T> *Wn,
                                            const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
                                            const cpx<T> *__restrict__ tw, DevScalars *sc,
-                                           const StageArgs &a, const Batch &bt)
+                                           const StageArgs &a, const Batch &bt, int tid = -1)
 {
     using C = WaveCfg<T, NP, V>;
-    const int half = threadIdx.x / C::BLK, lane = threadIdx.x % C::BLK;
+    // tid: the thread's index within the body's 2 * BLK threads (a block
+    // may hold two bodies: the ensemble's stage queue kernel)
+    const int th = tid < 0 ? (int)threadIdx.x : tid;
+    const int half = th / C::BLK, lane = th % C::BLK;
     // wave-uniform; both halves pass the same barriers
     if (half == 0)
         passC_half<T, NP, 0, TAIL, V>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);

@@ -101,12 +101,14 @@ class Ensemble:
         self._c(L.lib().s2d_ensemble_steps(self._h, int(nsteps), float(dt), float(cfl),
                                            L.ENS_GRAPH if graph else 0))
 
-    def set_schedule(self, per_class: bool = False, fused_a: bool = False):
-        """One launch per pass for all sizes (per_class False; fused_a: pass
-        A with the three fields of a column tile per block) or per size
-        class (True). Every schedule computes the same bits."""
-        code = L.ENS_PER_CLASS if per_class else (L.ENS_UNIFIED_FUSED_A if fused_a
-                                                   else L.ENS_UNIFIED)
+    def set_schedule(self, per_class: bool = False, fused_a: bool = False,
+                     stage_queue: bool = False):
+        """One launch per pass for all sizes (the default; fused_a: pass A
+        with the three fields of a column tile per block), per size class
+        (per_class), or one kernel per RK stage over a work queue (stage_queue).
+        Every schedule computes the same bits."""
+        code = (L.ENS_PER_CLASS if per_class else L.ENS_STAGE_QUEUE if stage_queue
+                else L.ENS_UNIFIED_FUSED_A if fused_a else L.ENS_UNIFIED)
         self._c(L.lib().s2d_ensemble_set_schedule(self._h, code))
 
     def tune(self, dt: float = 0.0, cfl: float = 0.5, cache: str | None = None,

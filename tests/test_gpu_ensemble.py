@@ -202,20 +202,21 @@ def test_c2_full_draw_members_equal_runs_alone():
 @pytest.mark.parametrize("precision", [FP32, FP64])
 def test_every_schedule_computes_the_same_bits(precision):
     """The unified launches, the unified launches with the fused pass A (the
-    three fields of a column tile in one block) and the per-class launches
-    give bitwise the same states, eager and by graph replay."""
+    three fields of a column tile in one block), the stage queue kernel
+    (per-run pass dependencies through a work queue) and the per-class
+    launches give bitwise the same states, eager and by graph replay."""
     sizes = [128, 64, 512, 256, 128, 512, 64]
     seeds, k0, nu = _members(sizes, seed0=1200)
     print(f"[ensemble] schedules, precision={precision} sizes={sizes}", flush=True)
     out = {}
     for name, kw in (("unified", {}), ("fused_a", {"fused_a": True}),
-                     ("per_class", {"per_class": True})):
+                     ("stage_queue", {"stage_queue": True}), ("per_class", {"per_class": True})):
         e = _ensemble(sizes, precision, seeds, k0, nu)
         e.set_schedule(**kw)
         e.steps(3, cfl=0.5)
         e.steps(4, cfl=0.5, graph=True)
         out[name] = [e.get_field(r, "w") for r in range(len(sizes))]
         e.close()
-    for name in ("fused_a", "per_class"):
+    for name in ("fused_a", "stage_queue", "per_class"):
         for r in range(len(sizes)):
             assert np.array_equal(out[name][r], out["unified"][r]), (name, r)
