@@ -5,9 +5,10 @@
 //
 // Network: layer 0 conv 2 -> 64, layers 1..4 conv 64 -> 64, layer 5
 // conv 64 -> 1; GELU (exact, erf) after layers 0..4. Activations are NHWC
-// BF16 (64 channels = 128 bytes per pixel). Layers 1..4 (97 % of the
-// arithmetic) run on the matrix cores (v_mfma_f32_16x16x16bf16_1k) as
-// implicit GEMMs; the thin first and last layers run on the vector ALUs.
+// BF16 (64 channels = 128 bytes per pixel). Layers 0..4 run on the matrix
+// cores (v_mfma_f32_16x16x32_bf16) as implicit GEMMs, forward, backward
+// data and weight gradient; the thin last layer and the thin layers'
+// weight gradients run on the vector ALUs.
 //
 // Rounding points (stated once here; the CPU oracle follows them): BF16 of
 // every weight, of the scaled inputs, of each layer's pre-activation z
@@ -19,6 +20,7 @@
 // ci), b5 [1]. Tap t = 3 dy + dx is the offset (dy - 1, dx - 1), periodic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -33,13 +35,11 @@
 namespace {
 
 using u16 = uint16_t;
-typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int CH = 64;       // channels of the hidden layers
 constexpr int NMID = 4;      // 64 -> 64 layers
-constexpr int TPX = 128;     // pixels per weight-gradient tile (one row segment)
-constexpr int LP = CH + 8;   // LDS pitch (BF16) of a staged pixel
+constexpr int WSEG = 128;    // W is a multiple of this (the layer-0 row segment)
 constexpr int W64 = 9 * CH * CH;
 
 constexpr long long OFF_W0 = 0, OFF_B0 = 9 * 2 * CH;
@@ -65,11 +65,31 @@ __host__ __device__ inline float bf2f(u16 h)
 }
 __device__ inline float bfr(float f) { return bf2f(f2bf(f)); }
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun 7.1.26, erf(x) = 1 - (a1 t + ... + a5 t^5)
+// exp(-x^2) with t = 1 / (1 + p |x|), odd in x; |error| <= 1.5e-7, far below
+// the BF16 rounding of every result it feeds. *g = exp(-x^2), which the
+// derivative reuses. (The hardware reciprocal and exp2 are within an ulp.)
+__device__ inline float erf_as(float x, float *g)
+{
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.3275911f, ax, 1.0f));
+    const float p =
+        t * (0.254829592f +
+             t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    *g = __builtin_amdgcn_exp2f(-1.44269504088896341f * (x * x));
+    return copysignf(1.0f - p * *g, x);
+}
+__device__ inline float gelu(float z)
+{
+    float g;
+    return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f, &g));
+}
+// gelu'(z) = Phi(z) + z phi(z), phi(z) = exp(-z^2 / 2) / sqrt(2 pi)
 __device__ inline float gelu_grad(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-           z * expf(-0.5f * z * z) * 0.39894228040143268f;
+    float g;
+    const float e = erf_as(z * 0.70710678118654752f, &g);
+    return 0.5f * (1.0f + e) + z * g * 0.39894228040143268f;
 }
 
 __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -78,70 +98,7 @@ __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - 
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
          i += (long long)gridDim.x * blockDim.x)
 
-// ------------------------------------------------------------ layer 0
-// Input planes: channel c of sample n at xc + n*sN, row pitch ld (floats).
-// Grid (W / 32, B * H): a block is 32 pixels of one row x 8 groups of 8
-// output channels.
-__global__ __launch_bounds__(256) void k_conv_in(const float *__restrict__ x0,
-                                                 const float *__restrict__ x1, long long sN,
-                                                 int ld, float s0, float s1,
-                                                 const float *__restrict__ P, u16 *__restrict__ z,
-                                                 u16 *__restrict__ a, int B, int H, int W)
-{
-    constexpr int NW = 9 * 2 * CH + CH, NWI = (NW + 255) / 256;
-    __shared__ float w[NW];
-    const int row = blockIdx.y, n = row / H, yy = row - n * H; // uniform
-    const int xx = blockIdx.x * 32 + (threadIdx.x >> 3), c8 = 8 * (threadIdx.x & 7);
-    // every load first (weights and the 18 inputs), then the arithmetic
-    float wv[NWI], i0[9], i1[9];
-#pragma unroll
-    for (int i = 0; i < NWI; ++i) {
-        const int j = threadIdx.x + 256 * i;
-        if (j < NW) wv[i] = j < 9 * 2 * CH ? P[OFF_W0 + j] : P[OFF_B0 + j - 9 * 2 * CH];
-    }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
-        const long long o = n * sN + (long long)y2 * ld + x2;
-        i0[t] = x0[o];
-        i1[t] = x1[o];
-    }
-#pragma unroll
-    for (int i = 0; i < NWI; ++i) {
-        const int j = threadIdx.x + 256 * i;
-        if (j < NW) w[j] = j < 9 * 2 * CH ? bfr(wv[i]) : wv[i];
-    }
-    __syncthreads();
-    float acc[8];
-    for (int k = 0; k < 8; ++k) acc[k] = w[9 * 2 * CH + c8 + k];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const float a0 = bfr(s0 * i0[t]), a1 = bfr(s1 * i1[t]);
-        for (int k = 0; k < 8; ++k)
-            acc[k] += a0 * w[(t * 2 + 0) * CH + c8 + k] + a1 * w[(t * 2 + 1) * CH + c8 + k];
-    }
-    u16 zo[8], ao[8];
-    for (int k = 0; k < 8; ++k) {
-        zo[k] = f2bf(acc[k]);
-        ao[k] = f2bf(gelu(acc[k]));
-    }
-    const size_t p = ((size_t)row * W + xx) * CH + c8;
-    *reinterpret_cast<uint4 *>(z + p) = *reinterpret_cast<const uint4 *>(zo);
-    *reinterpret_cast<uint4 *>(a + p) = *reinterpret_cast<const uint4 *>(ao);
-}
-
 // ------------------------------------------------- layers 1..4 (MFMA)
-// One block: a segment of CT = 64 MT pixels of one row x 64 output
-// channels; wave w computes pixels 16 MT w .. (MT 16-pixel M tiles) x 4 N
-// tiles of 16 channels. K = 9 taps x 64 input channels, in chunks of 16. The
-// input rows y-1..y+1 (with a one-pixel periodic halo) are staged in LDS.
-// Every thread issues its share of all nine taps' weights at the start (18
-// 16-byte loads in flight with the input staging: one memory latency per
-// block), then copies tap by tap into a double-buffered LDS tile.
-//   mode 0: out = acc + bias -> zo (BF16), gelu(acc + bias) -> ao (BF16)
-//   mode 1: out = acc * gelu'(zp) -> zo (BF16)   (the backward data pass:
-//           in = dZ_l, wt = the flipped transpose of W_l, zp = z_{l-1})
-// wt: BF16 [9][64 co][64 ci].
 // f(std::integral_constant<int, i>) for i = I..N-1, unrolled at compile time.
 template <int N, int I = 0, typename F>
 __device__ __forceinline__ void for_each_c(F &&f)
@@ -152,218 +109,415 @@ __device__ __forceinline__ void for_each_c(F &&f)
     }
 }
 
-template <int MT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv64(const u16 *__restrict__ in,
-                                                const u16 *__restrict__ wt,
-                                                const float *__restrict__ bias,
-                                                const u16 *__restrict__ zp, u16 *__restrict__ zo,
-                                                u16 *__restrict__ ao, int H, int W, int mode)
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ bf16x8_t as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8_t, v); }
+
+// Two floats to packed BF16, round to nearest even (v_cvt_pk_bf16_f32: the
+// rounding of f2bf for every non-NaN value).
+__device__ __forceinline__ uint32_t pk_bf16(float lo, float hi)
 {
-    constexpr int CT = 64 * MT; // pixels of the tile
-    __shared__ __align__(16) u16 tin[3 * (CT + 2) * LP];
-    __shared__ __align__(16) u16 tw[2][CH * LP];
-    const int segs = W / CT;
-    int b = blockIdx.x;
-    const int seg = b % segs;
-    b /= segs;
-    const int y = b % H, n = b / H;
-    const int x0 = seg * CT;
+    const bf16x2_t v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// LDS rows of 64 BF16 channels (eight 16-byte chunks): chunk c of row r is
+// stored at chunk c ^ ((r >> 1) & 7) of the row, so that the 16 rows an
+// operand read takes (16 lanes, one chunk each) fall on distinct banks.
+__device__ __forceinline__ int sw_chunk(int r, int c) { return r * 8 + (c ^ ((r >> 1) & 7)); }
+
+// ------------------------------------------------------------ layer 0
+// z0[p][co] = b0[co] + sum_t sum_ci W0[t][ci][co] xs_ci[p + off_t], on the
+// matrix cores: K = 9 taps x 2 input channels = 18, zero-padded to one
+// 32-wide step of v_mfma_f32_16x16x32_bf16 (k = 2 t + ci). A = the weights
+// [co][k] (held in registers), B = the scaled, BF16-rounded inputs [k][pixel]
+// (staged in LDS as (ci 0, ci 1) pairs per pixel). Input planes: channel c
+// of sample n at xc + n*sN, row pitch ld (floats).
+// One block: a 128-pixel row segment; wave w computes pixels 32 w .. 32 w + 31
+// (2 N tiles) x 64 channels (4 M tiles).
+constexpr int C0_SEG = 128;
+__global__ __launch_bounds__(256) void k_conv_in(const float *__restrict__ x0,
+                                                 const float *__restrict__ x1, long long sN,
+                                                 int ld, float s0, float s1,
+                                                 const float *__restrict__ P, u16 *__restrict__ z,
+                                                 u16 *__restrict__ a, int B, int H, int W)
+{
+    __shared__ uint32_t xs[3][C0_SEG + 2]; // (ci 0, ci 1) BF16 pairs, pixels x0 - 1 .. x0 + 128
+    const int segs = W / C0_SEG;
+    const int row = blockIdx.x / segs, seg = blockIdx.x % segs; // row = n H + y
+    const int n = row / H, yy = row - n * H, xb = seg * C0_SEG;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    uint4 wr[9][2]; // this thread's 16-byte chunks of every tap: q = tid + 256 i
+    const int l16 = lane & 15, g4 = lane >> 4;
+    for (int q = tid; q < 3 * (C0_SEG + 2); q += 256) {
+        const int r = q / (C0_SEG + 2), j = q % (C0_SEG + 2);
+        const int y2 = wrap(yy + r - 1, H), x2 = wrap(xb + j - 1, W);
+        const long long o = n * sN + (long long)y2 * ld + x2;
+        xs[r][j] = pk_bf16(s0 * x0[o], s1 * x1[o]);
+    }
+    // A: lane (co = 16 mt + l16, k = 8 g4 + e): W0[t][ci][co] for k = 2 t + ci < 18
+    bf16x8_t av[4];
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+    for (int mt = 0; mt < 4; ++mt) {
+        uint32_t w[4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int q = tid + 256 * i, co = q >> 3, chunk = q & 7;
-            wr[t][i] = *reinterpret_cast<const uint4 *>(wt + ((size_t)t * CH + co) * CH + 8 * chunk);
+        for (int e2 = 0; e2 < 4; ++e2) {
+            const int k = 8 * g4 + 2 * e2; // k and k + 1: tap k / 2, ci 0 and 1
+            const int co = 16 * mt + l16;
+            w[e2] = k < 18 ? pk_bf16(P[OFF_W0 + (k / 2 * 2 + 0) * CH + co],
+                                     P[OFF_W0 + (k / 2 * 2 + 1) * CH + co])
+                           : 0u;
         }
-    // the input rows: all loads of the thread in flight together, then the
-    // LDS stores
-    constexpr int NST = 3 * (CT + 2) * 8, NSI = (NST + 255) / 256;
-    uint4 st[NSI];
-    for_each_c<NSI>([&](auto iI) {
-        constexpr int i = decltype(iI)::value;
-        const int q = tid + 256 * i;
-        st[i] = uint4{0, 0, 0, 0};
-        if (q < NST) {
-            const int chunk = q & 7, pr = q >> 3;
-            const int r = pr / (CT + 2), px = pr % (CT + 2);
-            const int yy = wrap(y + r - 1, H), xx = wrap(x0 + px - 1, W);
-            st[i] = *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CH +
-                                                     8 * chunk);
-        }
-    });
-    for_each_c<NSI>([&](auto iI) {
-        constexpr int i = decltype(iI)::value;
-        const int q = tid + 256 * i;
-        if (q < NST) *reinterpret_cast<uint4 *>(tin + (q >> 3) * LP + 8 * (q & 7)) = st[i];
-    });
-    // thread tid holds chunks q = tid and tid + 256: co = q >> 3 (and co + 32)
-    const int wco = tid >> 3, wch = tid & 7;
-    auto store_w = [&](uint4 lo, uint4 hi, int buf) {
-        *reinterpret_cast<uint4 *>(tw[buf] + wco * LP + 8 * wch) = lo;
-        *reinterpret_cast<uint4 *>(tw[buf] + (wco + 32) * LP + 8 * wch) = hi;
-    };
-    store_w(wr[0][0], wr[0][1], 0);
+        av[mt] = as_bf8(uint4{w[0], w[1], w[2], w[3]});
+    }
     __syncthreads();
-    float4_t acc[MT][4];
-    for (int m = 0; m < MT; ++m)
-        for (int j = 0; j < 4; ++j) acc[m][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    const int row16 = lane & 15, kq = 4 * (lane >> 4);
-    // taps at compile time, so that wr stays in registers
-    for_each_c<9>([&](auto tI) {
-        constexpr int t = decltype(tI)::value;
-        if constexpr (t + 1 < 9) store_w(wr[t + 1][0], wr[t + 1][1], (t + 1) & 1);
-        constexpr int dy = t / 3, dx = t % 3;
-        const u16 *wb = tw[t & 1];
+    // B: lane (pixel 32 w + 16 nt + l16, k = 8 g4 + e): the pair of tap (8 g4 + e) / 2
+    float4_t acc[4][2];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int kk = 16 * c + kq;
-            short4_t av[MT], bv[4];
+    for (int nt = 0; nt < 2; ++nt) {
+        const int j = 32 * wv + 16 * nt + l16; // pixel xb + j
+        uint32_t w[4];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const int px = 16 * MT * wv + 16 * m + row16 + dx;
-                av[m] = *reinterpret_cast<const short4_t *>(tin + (dy * (CT + 2) + px) * LP + kk);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                bv[j] = *reinterpret_cast<const short4_t *>(wb + (16 * j + row16) * LP + kk);
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[j], acc[m][j],
-                                                                          0, 0, 0);
+        for (int e2 = 0; e2 < 4; ++e2) {
+            const int t = 4 * g4 + e2; // the tap of k = 8 g4 + 2 e2
+            w[e2] = t < 9 ? xs[t / 3][j + t % 3] : 0u;
         }
-        __syncthreads();
-    });
-    // results through LDS (the input tile is free): [px][co] FP32, pitch 68
-    constexpr int SOP = CH + 4;
-    float *so = reinterpret_cast<float *>(tin);
-    for (int m = 0; m < MT; ++m)
-        for (int j = 0; j < 4; ++j)
-            for (int e = 0; e < 4; ++e) {
-                const int px = 16 * MT * wv + 16 * m + kq + e, co = 16 * j + row16;
-                so[px * SOP + co] = acc[m][j][e] + (bias ? bias[co] : 0.f);
-            }
-    __syncthreads();
-    for (int q = tid; q < CT * 8; q += 256) {
-        const int px = q >> 3, c0 = 8 * (q & 7);
-        const size_t gi = (((size_t)n * H + y) * W + x0 + px) * CH + c0;
-        u16 o1[8], o2[8];
-        if (mode == 0) {
-            for (int k = 0; k < 8; ++k) {
-                const float v = so[px * SOP + c0 + k];
-                o1[k] = f2bf(v);
-                o2[k] = f2bf(gelu(v));
-            }
-            *reinterpret_cast<uint4 *>(zo + gi) = *reinterpret_cast<const uint4 *>(o1);
-            *reinterpret_cast<uint4 *>(ao + gi) = *reinterpret_cast<const uint4 *>(o2);
-        } else {
-            const uint4 zz = *reinterpret_cast<const uint4 *>(zp + gi);
-            const u16 *zh = reinterpret_cast<const u16 *>(&zz);
-            for (int k = 0; k < 8; ++k) o1[k] = f2bf(so[px * SOP + c0 + k] * gelu_grad(bf2f(zh[k])));
-            *reinterpret_cast<uint4 *>(zo + gi) = *reinterpret_cast<const uint4 *>(o1);
+        const bf16x8_t bv = as_bf8(uint4{w[0], w[1], w[2], w[3]});
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                av[mt], bv, float4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
+    // lane: channels co = 16 mt + 4 g4 + i of pixel xb + 32 w + 16 nt + l16
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const size_t pix = ((size_t)row * W + xb + 32 * wv + 16 * nt + l16) * CH + 4 * g4;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const float4 bb = *reinterpret_cast<const float4 *>(P + OFF_B0 + 16 * mt + 4 * g4);
+            const float v[4] = {acc[mt][nt][0] + bb.x, acc[mt][nt][1] + bb.y,
+                                acc[mt][nt][2] + bb.z, acc[mt][nt][3] + bb.w};
+            const size_t o = pix + 16 * mt;
+            if (z) *reinterpret_cast<uint2 *>(z + o) = uint2{pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3])};
+            *reinterpret_cast<uint2 *>(a + o) =
+                uint2{pk_bf16(gelu(v[0]), gelu(v[1])), pk_bf16(gelu(v[2]), gelu(v[3]))};
         }
     }
 }
 
-// Weight gradient of a 64 -> 64 layer on the matrix cores:
-// dW[t][co][ci] = sum_p dZ[p][co] a[p + off_t][ci], db[co] = sum_p dZ[p][co].
-// One block: ROWS rows of one sample, segment by segment; wave w owns the
-// output rows co in [16w, 16w + 16) for all 9 taps and 4 ci tiles
-// (144 accumulator registers). K = pixels: both operands are staged
-// transposed ([channel][pixel]); the tap's x shift makes the input operand's
-// four pixels start at any 2-byte offset, so it is read element by element.
-// Writes per-block partial sums (reduced by k_reduce).
-constexpr int DW_ROWS = 8;
-constexpr int TP = TPX + 8; // pitch of a transposed pixel row
+// One block: R rows (a multiple of 8) of a 64-pixel column segment of one
+// sample, all 64 output channels. The nine taps' weights stay in LDS for
+// the whole block (72 KiB); the input rows go through a ring of ten LDS
+// rows (64 + 2 halo pixels each). A step makes eight output rows, one per
+// wave: wave w's row is a 64 (co) x 64 (pixel) tile, 4 x 4 MFMA tiles of
+// v_mfma_f32_16x16x32_bf16 (A = weights [co][ci], B = input [pixel][ci];
+// K = 9 taps x 64 ci in chunks of 32). The eight input rows of the next
+// step are loaded into registers during the step and written over the ring
+// rows it no longer needs.
+//   mode 0: out = acc + bias -> zo (BF16; skipped when zo is null),
+//           gelu(acc + bias) -> ao (BF16)
+//   mode 1: out = acc * gelu'(zp) -> zo (BF16)   (the backward data pass:
+//           in = dZ_l, wt = the flipped transpose of W_l, zp = z_{l-1})
+// wt: BF16 [9][64 co][64 ci]. Output stores: each lane writes four
+// channels (8 bytes) of one pixel per tile.
+constexpr int CV_SEG = 64, CV_PX = CV_SEG + 2, CV_RING = 10, CV_THREADS = 512;
+constexpr int CV_ROWCH = CV_PX * 8; // 16-byte chunks of a ring row
 
-__device__ inline short4_t ld4_any(const u16 *p)
+__global__ __launch_bounds__(CV_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_conv64(
+    const u16 *__restrict__ in, const u16 *__restrict__ wt, const float *__restrict__ bias,
+    const u16 *__restrict__ zp, u16 *__restrict__ zo, u16 *__restrict__ ao, int H, int W, int R,
+    int mode)
 {
-    short4_t r;
-    r[0] = (short)p[0];
-    r[1] = (short)p[1];
-    r[2] = (short)p[2];
-    r[3] = (short)p[3];
-    return r;
+    __shared__ uint4 sw[9 * CH * 8];             // 72 KiB
+    __shared__ uint4 sr[CV_RING * CV_ROWCH];     // 82.5 KiB
+    const int segs = W / CV_SEG, nyb = H / R;
+    int b = blockIdx.x;
+    const int seg = b % segs;
+    b /= segs;
+    const int yb = b % nyb, n = b / nyb;
+    const int x0 = seg * CV_SEG, y0 = yb * R, S = R / 8;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, g4 = lane >> 4;
+    const uint4 *gin = reinterpret_cast<const uint4 *>(in);
+    // chunk q of ring row i (input row y0 - 1 + i)
+    auto row_chunk = [&](int i, int q) {
+        const int j = q >> 3, c = q & 7;
+        const int yy = wrap(y0 - 1 + i, H), xx = wrap(x0 + j - 1, W);
+        return gin[(((size_t)n * H + yy) * W + xx) * 8 + c];
+    };
+    {
+        // the weights, then the first ten rows (each: the loads, then the stores)
+        const uint4 *gw = reinterpret_cast<const uint4 *>(wt);
+        constexpr int NWC = 9 * CH * 8 / CV_THREADS; // 9
+        constexpr int NRC = (CV_RING * CV_ROWCH + CV_THREADS - 1) / CV_THREADS; // 11
+        uint4 vw[NWC], vr[NRC];
+#pragma unroll
+        for (int k = 0; k < NWC; ++k) vw[k] = gw[tid + CV_THREADS * k];
+#pragma unroll
+        for (int k = 0; k < NWC; ++k) {
+            const int q = tid + CV_THREADS * k;
+            sw[sw_chunk(q >> 3, q & 7)] = vw[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NRC; ++k) {
+            const int q = tid + CV_THREADS * k;
+            if (q < CV_RING * CV_ROWCH) vr[k] = row_chunk(q / CV_ROWCH, q % CV_ROWCH);
+        }
+#pragma unroll
+        for (int k = 0; k < NRC; ++k) {
+            const int q = tid + CV_THREADS * k;
+            if (q < CV_RING * CV_ROWCH) {
+                const int i = q / CV_ROWCH, r = q % CV_ROWCH;
+                sr[sw_chunk(i * CV_PX + (r >> 3), r & 7)] = vr[k];
+            }
+        }
+    }
+    __syncthreads();
+    // Waves 0-3 (group 0) and 4-7 (group 1) share the SIMDs one to one and run
+    // in opposite phases, so that one wave's MFMAs overlap the other's
+    // epilogue (VALU) on every SIMD. Step s, output rows y0 + 8 s + 4 grp + wr:
+    //   phase 1: group 0 MFMAs; group 1 the epilogue of its rows of step s - 1
+    //            and the loads of ring rows 8 s + 10 .. 8 s + 13
+    //   phase 2: group 1 stores those rows, then its MFMAs; group 0 its
+    //            epilogue and the loads of ring rows 8 s + 14 .. 8 s + 17,
+    //            which it stores at the start of step s + 1.
+    // Ring rows 8 s + 10 .. 13 take the slots of rows 8 s .. 8 s + 3 (last read
+    // by group 0 in phase 1), rows 8 s + 14 .. 17 those of 8 s + 4 .. 8 s + 7
+    // (last read by group 1 in phase 2); a barrier ends each phase.
+    const int grp = wv >> 2, wr = wv & 3, gt = tid & 255;
+    constexpr int NPF = (4 * CV_ROWCH + 255) / 256; // 9 chunks per thread of a group
+    uint4 pf[NPF];
+    auto load_rows = [&](int i0) { // ring rows i0 .. i0 + 3, the group's 256 threads
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) {
+            const int q = gt + 256 * k;
+            if (q < 4 * CV_ROWCH) pf[k] = row_chunk(i0 + q / CV_ROWCH, q % CV_ROWCH);
+        }
+    };
+    auto store_rows = [&](int i0) {
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) {
+            const int q = gt + 256 * k;
+            if (q < 4 * CV_ROWCH) {
+                const int i = i0 + q / CV_ROWCH, r = q % CV_ROWCH;
+                sr[sw_chunk((i % CV_RING) * CV_PX + (r >> 3), r & 7)] = pf[k];
+            }
+        }
+    };
+    float4_t acc[4][4];
+    // the output row of ring row i0 + 1 (input rows i0 .. i0 + 2)
+    auto mfma_row = [&](int i0) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+        for_each_c<9>([&](auto tI) {
+            constexpr int t = decltype(tI)::value, dy = t / 3, dx = t % 3;
+            const int base = ((i0 + dy) % CV_RING) * CV_PX + dx + l16;
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc) {
+                bf16x8_t av[4], bv[4];
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+                    av[mt] = as_bf8(sw[sw_chunk(t * CH + 16 * mt + l16, 4 * kc + g4)]);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    bv[nt] = as_bf8(sr[sw_chunk(base + 16 * nt, 4 * kc + g4)]);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[mt], bv[nt],
+                                                                              acc[mt][nt], 0, 0, 0);
+            }
+        });
+    };
+    // lane: channels co = 16 mt + 4 g4 + i of pixel x0 + 16 nt + l16 of image row y
+    auto epilogue = [&](int y) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const size_t pix = (((size_t)n * H + y) * W + x0 + 16 * nt + l16) * CH + 4 * g4;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const size_t o = pix + 16 * mt;
+                float4 bb = float4{0.f, 0.f, 0.f, 0.f};
+                if (bias) bb = *reinterpret_cast<const float4 *>(bias + 16 * mt + 4 * g4);
+                const float v[4] = {acc[mt][nt][0] + bb.x, acc[mt][nt][1] + bb.y,
+                                    acc[mt][nt][2] + bb.z, acc[mt][nt][3] + bb.w};
+                if (mode == 0) {
+                    if (zo)
+                        *reinterpret_cast<uint2 *>(zo + o) =
+                            uint2{pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3])};
+                    *reinterpret_cast<uint2 *>(ao + o) =
+                        uint2{pk_bf16(gelu(v[0]), gelu(v[1])), pk_bf16(gelu(v[2]), gelu(v[3]))};
+                } else {
+                    const uint2 zz = *reinterpret_cast<const uint2 *>(zp + o);
+                    const u16 *zh = reinterpret_cast<const u16 *>(&zz);
+                    float g[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) g[i] = v[i] * gelu_grad(bf2f(zh[i]));
+                    *reinterpret_cast<uint2 *>(zo + o) =
+                        uint2{pk_bf16(g[0], g[1]), pk_bf16(g[2], g[3])};
+                }
+            }
+        }
+    };
+    for (int s = 0; s < S; ++s) {
+        const bool more = s + 1 < S;
+        // phase 1
+        if (grp == 0) {
+            if (s > 0) store_rows(8 * s + 6); // rows 8 (s - 1) + 14 .. 17
+            mfma_row(8 * s + wr);
+        } else {
+            if (s > 0) epilogue(y0 + 8 * (s - 1) + 4 + wr);
+            if (more) load_rows(8 * s + 10);
+        }
+        __syncthreads();
+        // phase 2
+        if (grp == 1) {
+            if (more) store_rows(8 * s + 10);
+            mfma_row(8 * s + 4 + wr);
+        } else {
+            epilogue(y0 + 8 * s + wr);
+            if (more) load_rows(8 * s + 14);
+        }
+        __syncthreads();
+    }
+    if (grp == 1) epilogue(y0 + 8 * (S - 1) + 4 + wr);
 }
 
-__global__ __launch_bounds__(256) void k_dw64(const u16 *__restrict__ dz,
-                                              const u16 *__restrict__ a, int H, int W,
-                                              float *__restrict__ part)
+// Weight gradient of a 64 -> 64 layer on the matrix cores:
+// dW[t][co][ci] = sum_p dZ[p][co] a[p + off_t][ci] = sum_q dZ[q - off_t][co] a[q][ci],
+// db[co] = sum_p dZ[p][co]. K = pixels, so both operands are staged
+// channel-major ([channel][pixel], 16-byte chunks of 8 pixels); the tap
+// shift goes on dZ, staged once per row in three x-shifted copies, so every
+// operand read is an aligned chunk.
+// One block: a 32-pixel column strip of R rows of one sample (rows q of a).
+// Row step q: the MFMA K is the strip's 32 pixels; wave w owns co tile w
+// for the 9 taps x 4 ci tiles (36 accumulator tiles). dZ rows q - 1 .. q + 1
+// sit in a ring of four (three x-shifts each), a's row q in one of two
+// buffers; the next step's rows are loaded into registers during the step.
+// Writes per-block partial sums (reduced by k_reduce).
+constexpr int DW_SEG = 32, DW_THREADS = 256;
+
+// chunk position of [row r][chunk c] in a channel-major LDS row of four
+// chunks (conflict-free operand reads of 16 consecutive rows)
+__device__ __forceinline__ int dw_chunk(int r, int c) { return r * 4 + (c ^ ((r >> 2) & 3)); }
+
+__global__ __launch_bounds__(DW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_dw64(
+    const u16 *__restrict__ dz, const u16 *__restrict__ a, int H, int W, int R,
+    float *__restrict__ part)
 {
-    __shared__ __align__(16) u16 zt[CH * TP];          // dZ [co][px]
-    __shared__ __align__(16) u16 at[CH * 3 * TP];      // a [ci][dy][px'] (px' = px + 1)
-    __shared__ float dbs[256 * 8];
-    const int rb = H / DW_ROWS;
-    const int n = blockIdx.x / rb, y0 = (blockIdx.x % rb) * DW_ROWS;
+    __shared__ uint4 zt[4 * 3 * CH * 4]; // [ring slot][dx][co][4 chunks]: 48 KiB
+    __shared__ uint4 at[2 * CH * 4];     // [buffer][ci][4 chunks]: 8 KiB
+    __shared__ float dbs[4][CH];
+    const int segs = W / DW_SEG, nyb = H / R;
+    int b = blockIdx.x;
+    const int seg = b % segs;
+    b /= segs;
+    const int yb = b % nyb, n = b / nyb;
+    const int x0 = seg * DW_SEG, q0 = yb * R;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int row16 = lane & 15, kq = 4 * (lane >> 4);
+    const int l16 = lane & 15, g4 = lane >> 4;
+    // staging roles: channel ch = lane, pixel group jg = wave (pixels 8 jg .. 8 jg + 7)
+    const int ch = lane, jg = wv;
+    // dZ row k of the ring (image row q0 - 1 + k): the ten pixels
+    // x0 + 8 jg - 1 .. x0 + 8 jg + 8 of channel ch
+    auto load_dz = [&](int k, u16 (&v)[10]) {
+        const size_t rb = ((size_t)n * H + wrap(q0 - 1 + k, H)) * W;
+#pragma unroll
+        for (int m = 0; m < 10; ++m) v[m] = dz[(rb + wrap(x0 + 8 * jg - 1 + m, W)) * CH + ch];
+    };
+    // zt[dx][ch][8 jg + i] = dZ[x0 + 8 jg + i - dx + 1] = v[i - dx + 2]
+    auto store_dz = [&](int k, const u16 (&v)[10]) {
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            uint32_t w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                w[i] = (uint32_t)v[2 * i + 2 - dx] | ((uint32_t)v[2 * i + 3 - dx] << 16);
+            zt[((k & 3) * 3 + dx) * CH * 4 + dw_chunk(ch, jg)] = uint4{w[0], w[1], w[2], w[3]};
+        }
+    };
+    auto load_a = [&](int q, u16 (&v)[8]) {
+        const size_t rb = ((size_t)n * H + q) * W + x0 + 8 * jg;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = a[(rb + m) * CH + ch];
+    };
+    auto store_a = [&](int buf, const u16 (&v)[8]) {
+        uint32_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = (uint32_t)v[2 * i] | ((uint32_t)v[2 * i + 1] << 16);
+        at[buf * CH * 4 + dw_chunk(ch, jg)] = uint4{w[0], w[1], w[2], w[3]};
+    };
+    float dsum = 0.f; // db: the strip's own pixels of rows q0 .. q0 + R - 1 (ring rows 1 .. R)
+    auto add_db = [&](int k, const u16 (&v)[10]) {
+        if (k >= 1 && k <= R)
+#pragma unroll
+            for (int m = 1; m <= 8; ++m) dsum += bf2f(v[m]);
+    };
+    {
+        u16 v0[10], v1[10], v2[10], va[8];
+        load_dz(0, v0);
+        load_dz(1, v1);
+        load_dz(2, v2);
+        load_a(q0, va);
+        store_dz(0, v0);
+        store_dz(1, v1);
+        store_dz(2, v2);
+        store_a(0, va);
+        add_db(0, v0);
+        add_db(1, v1);
+        add_db(2, v2);
+    }
+    __syncthreads();
     float4_t acc[9][4];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[t][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    float dsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int y = y0; y < y0 + DW_ROWS; ++y)
-        for (int x0 = 0; x0 < W; x0 += TPX) {
-            __syncthreads();
-            for (int q = tid; q < TPX * 8; q += 256) { // dZ, transposed
-                const int px = q >> 3, c0 = 8 * (q & 7);
-                const uint4 v = *reinterpret_cast<const uint4 *>(
-                    dz + (((size_t)n * H + y) * W + x0 + px) * CH + c0);
-                const u16 *h = reinterpret_cast<const u16 *>(&v);
-                for (int k = 0; k < 8; ++k) {
-                    zt[(c0 + k) * TP + px] = h[k];
-                    dsum[k] += bf2f(h[k]); // q & 7 == tid & 7: fixed channels per thread
-                }
-            }
-            for (int q = tid; q < 3 * (TPX + 2) * 8; q += 256) { // a, transposed
-                const int c0 = 8 * (q & 7), pr = q >> 3;
-                const int r = pr / (TPX + 2), px = pr % (TPX + 2);
-                const int yy = wrap(y + r - 1, H), xx = wrap(x0 + px - 1, W);
-                const uint4 v = *reinterpret_cast<const uint4 *>(
-                    a + (((size_t)n * H + yy) * W + xx) * CH + c0);
-                const u16 *h = reinterpret_cast<const u16 *>(&v);
-                for (int k = 0; k < 8; ++k) at[((c0 + k) * 3 + r) * TP + px] = h[k];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                const int dy = t / 3, dx = t % 3;
-                for (int ks = 0; ks < TPX / 16; ++ks) {
-                    const int p0 = 16 * ks + kq;
-                    const short4_t av =
-                        *reinterpret_cast<const short4_t *>(zt + (16 * wv + row16) * TP + p0);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const u16 *src = at + ((16 * j + row16) * 3 + dy) * TP + p0 + dx;
-                        const short4_t bv = dx == 0 ? *reinterpret_cast<const short4_t *>(src)
-                                                    : ld4_any(src);
-                        acc[t][j] =
-                            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t][j], 0, 0, 0);
-                    }
-                }
-            }
+    for (int i = 0; i < R; ++i) {
+        // the next step's rows: dZ ring row i + 3, a row q0 + i + 1
+        const bool more = i + 1 < R;
+        u16 nz[10], na[8];
+        if (more) {
+            load_dz(i + 3, nz);
+            load_a(q0 + i + 1, na);
         }
-    // partial: D[co][ci] per tap; lane holds rows co = 16w + kq + e, col ci = 16j + row16
+        bf16x8_t bv[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            bv[nt] = as_bf8(at[(i & 1) * CH * 4 + dw_chunk(16 * nt + l16, g4)]);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3; // dZ row q - dy + 1 = ring row i + 2 - dy
+            const bf16x8_t av = as_bf8(
+                zt[(((i + 2 - dy) & 3) * 3 + dx) * CH * 4 + dw_chunk(16 * wv + l16, g4)]);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nt], acc[t][nt], 0, 0, 0);
+        }
+        // ring row i + 3 takes the slot of row i - 1 and buffer (i + 1) & 1 that
+        // of a row q0 + i - 1: both were last read in the previous step
+        if (more) {
+            store_dz(i + 3, nz);
+            store_a((i + 1) & 1, na);
+            add_db(i + 3, nz);
+        }
+        __syncthreads();
+    }
+    // partial: D[co][ci] per tap; lane holds co = 16 w + 4 g4 + e, ci = 16 nt + l16
     float *pb = part + (size_t)blockIdx.x * (W64 + CH);
 #pragma unroll
     for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            for (int e = 0; e < 4; ++e) {
-                const int co = 16 * wv + kq + e, ci = 16 * j + row16;
-                pb[((size_t)t * CH + co) * CH + ci] = acc[t][j][e];
-            }
-    for (int k = 0; k < 8; ++k) dbs[tid * 8 + k] = dsum[k];
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                pb[((size_t)t * CH + 16 * wv + 4 * g4 + e) * CH + 16 * nt + l16] = acc[t][nt][e];
+    dbs[jg][ch] = dsum;
     __syncthreads();
-    if (tid < CH) { // channel tid: threads with (thread & 7) == tid / 8, element tid % 8
-        float s = 0.f;
-        for (int th = tid / 8; th < 256; th += 8) s += dbs[th * 8 + tid % 8];
-        pb[W64 + tid] = s;
-    }
+    if (tid < CH) pb[W64 + tid] = dbs[0][tid] + dbs[1][tid] + dbs[2][tid] + dbs[3][tid];
 }
 
 // ------------------------------------------------------------ layer 5
@@ -451,104 +605,172 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y,
 }
 
 // Backward of layer 5: dZ4[q][ci] = (sum_t dy[q - off_t] W5[t][ci]) *
-// gelu'(z4[q][ci]) (BF16).
+// gelu'(z4[q][ci]) (BF16). Eight lanes per pixel, eight channels each; a
+// lane keeps its channels' 72 weights in registers over a grid-stride loop.
 __global__ __launch_bounds__(256) void k_out_bwd_data(const float *__restrict__ dy,
                                                       const float *__restrict__ P,
                                                       const u16 *__restrict__ z4,
                                                       u16 *__restrict__ dz, int B, int H, int W)
 {
-    __shared__ float w[9 * CH];
-    for (int i = threadIdx.x; i < 9 * CH; i += blockDim.x) w[i] = bfr(P[OFF_W5 + i]);
-    __syncthreads();
+    const int c8 = 8 * (threadIdx.x & 7);
+    float w[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[t][k] = bfr(P[OFF_W5 + t * CH + c8 + k]);
     const long long npx = (long long)B * H * W;
-    S2D_GRID_STRIDE(q, npx)
-    {
+    for (long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 3; q < npx;
+         q += ((long long)gridDim.x * blockDim.x) >> 3) {
         const int xx = (int)(q % W), yy = (int)((q / W) % H), n = (int)(q / ((long long)W * H));
+        const uint4 zz = *reinterpret_cast<const uint4 *>(z4 + q * CH + c8);
         float d[9];
+#pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int y2 = wrap(yy - (t / 3 - 1), H), x2 = wrap(xx - (t % 3 - 1), W);
             d[t] = dy[((size_t)n * H + y2) * W + x2];
         }
-        for (int c8 = 0; c8 < CH; c8 += 8) {
-            const uint4 zz = *reinterpret_cast<const uint4 *>(z4 + q * CH + c8);
-            const u16 *zh = reinterpret_cast<const u16 *>(&zz);
-            u16 o[8];
-            for (int k = 0; k < 8; ++k) {
-                float s = 0.f;
-                for (int t = 0; t < 9; ++t) s += d[t] * w[t * CH + c8 + k];
-                o[k] = f2bf(s * gelu_grad(bf2f(zh[k])));
-            }
-            *reinterpret_cast<uint4 *>(dz + q * CH + c8) = *reinterpret_cast<const uint4 *>(o);
+        const u16 *zh = reinterpret_cast<const u16 *>(&zz);
+        float o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float s = 0.f;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) s += d[t] * w[t][k];
+            o[k] = s * gelu_grad(bf2f(zh[k]));
         }
+        *reinterpret_cast<uint4 *>(dz + q * CH + c8) =
+            uint4{pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]), pk_bf16(o[4], o[5]),
+                  pk_bf16(o[6], o[7])};
     }
 }
 
-// Weight gradient of layer 5: one block per (sample, row band); thread j in
-// [0, 576) owns (t, ci) = (j / 64, j % 64); thread 576 owns db5. Partial
-// sums per block (reduced by k_reduce).
-constexpr int OUT_ROWS = 8;
-__global__ __launch_bounds__(640) void k_out_bwd_w(const float *__restrict__ dy,
-                                                   const u16 *__restrict__ a, int H, int W,
+// Weight gradients of the thin layers: one block per 64-pixel column
+// segment of R rows of one sample; thread (channel = lane, pixel group =
+// wave) walks 16 pixels of each row, accumulates in FP32, and the four
+// waves' sums are added in LDS. Per-block partial sums, reduced by k_reduce.
+// R: GR, doubled (dividing H) while at least 512 blocks remain.
+constexpr int GR = 8, GSEG = 64;
+int thin_rows(int B, int H, int W)
+{
+    int R = GR;
+    while (H % (2 * R) == 0 && (long long)B * (H / (2 * R)) * (W / GSEG) >= 512) R *= 2;
+    return R;
+}
+int thin_blocks(int B, int H, int W) { return B * (H / thin_rows(B, H, W)) * (W / GSEG); }
+
+// Layer 5: dW5[t][ci] = sum_p dy[p] a4[p + off_t][ci], db5 = sum_p dy[p];
+// partial layout [t][ci] then db5.
+__global__ __launch_bounds__(256) void k_out_bwd_w(const float *__restrict__ dy,
+                                                   const u16 *__restrict__ a, int H, int W, int R,
                                                    float *__restrict__ part)
 {
-    const int rb = H / OUT_ROWS;
-    const int n = blockIdx.x / rb, y0 = (blockIdx.x % rb) * OUT_ROWS;
-    const int j = threadIdx.x;
-    if (j > 9 * CH) return;
-    const int t = j / CH, ci = j % CH;
-    float s = 0.f;
-    for (int yy = y0; yy < y0 + OUT_ROWS; ++yy)
-        for (int xx = 0; xx < W; ++xx) {
-            const float d = dy[((size_t)n * H + yy) * W + xx];
-            if (j == 9 * CH) {
-                s += d;
-            } else {
-                const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
-                s += d * bf2f(a[(((size_t)n * H + y2) * W + x2) * CH + ci]);
+    __shared__ float red[4][9 * CH + 1];
+    const int segs = W / GSEG, rb = H / R;
+    const int seg = blockIdx.x % segs, r = (blockIdx.x / segs) % rb, n = blockIdx.x / (segs * rb);
+    const int ci = threadIdx.x & 63, jg = threadIdx.x >> 6;
+    const int xs0 = seg * GSEG + 16 * jg;
+    float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sb = 0.f;
+    for (int yy = r * R; yy < r * R + R; ++yy) {
+        const size_t rows[3] = {((size_t)n * H + wrap(yy - 1, H)) * W, ((size_t)n * H + yy) * W,
+                                ((size_t)n * H + wrap(yy + 1, H)) * W};
+        // window: v[dy][dx] = a[yy + dy - 1][xx + dx - 1][ci]
+        float v[3][3];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx)
+                v[dy][dx + 1] = bf2f(a[(rows[dy] + wrap(xs0 + dx - 1, W)) * CH + ci]);
+        for (int j = 0; j < 16; ++j) {
+            const int xx = xs0 + j;
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                v[dy][0] = v[dy][1];
+                v[dy][1] = v[dy][2];
+                v[dy][2] = bf2f(a[(rows[dy] + wrap(xx + 1, W)) * CH + ci]);
             }
+            const float d = dy[(size_t)n * H * W + (size_t)yy * W + xx];
+            sb += d;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) s[t] += d * v[t / 3][t % 3];
         }
-    part[(size_t)blockIdx.x * (9 * CH + 1) + j] = s;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) red[jg][t * CH + ci] = s[t];
+    if (ci == 0) red[jg][9 * CH] = sb;
+    __syncthreads();
+    float *pb = part + (size_t)blockIdx.x * (9 * CH + 1);
+    for (int i = threadIdx.x; i < 9 * CH + 1; i += 256)
+        pb[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
 }
 
-// Weight gradient of layer 0: thread (g, co) of 192 owns the six (t, ci)
-// pairs 6g .. 6g + 5 (pair = 2 t + ci) and, for g == 0, db0[co].
-__global__ __launch_bounds__(192) void k_in_bwd_w(const float *__restrict__ x0,
+// Layer 0: dW0[t][ci][co] = sum_p xs_ci[p + off_t] dZ0[p][co] (xs the scaled,
+// BF16-rounded input), db0[co] = sum_p dZ0[p][co]; partial layout
+// [(t, ci)][co] then db0. The inputs go through LDS GR rows at a time.
+__global__ __launch_bounds__(256) void k_in_bwd_w(const float *__restrict__ x0,
                                                   const float *__restrict__ x1, long long sN,
                                                   int ld, float s0, float s1,
-                                                  const u16 *__restrict__ dz, int H, int W,
+                                                  const u16 *__restrict__ dz, int H, int W, int R,
                                                   float *__restrict__ part)
 {
-    const int rb = H / OUT_ROWS;
-    const int n = blockIdx.x / rb, y0 = (blockIdx.x % rb) * OUT_ROWS;
-    const int co = threadIdx.x % CH, g = threadIdx.x / CH;
-    float s[6] = {0, 0, 0, 0, 0, 0}, sb = 0.f;
-    for (int yy = y0; yy < y0 + OUT_ROWS; ++yy)
-        for (int xx = 0; xx < W; ++xx) {
-            const float d = bf2f(dz[(((size_t)n * H + yy) * W + xx) * CH + co]);
-            sb += d;
-            for (int k = 0; k < 6; ++k) {
-                const int pr = 6 * g + k, t = pr / 2, ci = pr % 2;
-                const int y2 = wrap(yy + t / 3 - 1, H), x2 = wrap(xx + t % 3 - 1, W);
-                const float v = (ci == 0 ? x0 : x1)[n * sN + (long long)y2 * ld + x2];
-                s[k] += bfr((ci == 0 ? s0 : s1) * v) * d;
-            }
+    constexpr int NO = 9 * 2 * CH + CH;
+    __shared__ float xs[2][GR + 2][GSEG + 2];
+    __shared__ float red[4][NO];
+    const int segs = W / GSEG, rb = H / R;
+    const int seg = blockIdx.x % segs, r = (blockIdx.x / segs) % rb, n = blockIdx.x / (segs * rb);
+    const int co = threadIdx.x & 63, jg = threadIdx.x >> 6;
+    const int xb = seg * GSEG;
+    float s[9][2], sb = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) s[t][0] = s[t][1] = 0.f;
+    for (int y0 = r * R; y0 < r * R + R; y0 += GR) {
+        __syncthreads(); // the previous rows' reads are done
+        for (int q = threadIdx.x; q < (GR + 2) * (GSEG + 2); q += 256) {
+            const int i = q / (GSEG + 2), j = q % (GSEG + 2);
+            const long long o = n * sN + (long long)wrap(y0 + i - 1, H) * ld + wrap(xb + j - 1, W);
+            xs[0][i][j] = bfr(s0 * x0[o]);
+            xs[1][i][j] = bfr(s1 * x1[o]);
         }
-    float *pb = part + (size_t)blockIdx.x * (9 * 2 * CH + CH);
-    for (int k = 0; k < 6; ++k) {
-        const int pr = 6 * g + k, t = pr / 2, ci = pr % 2;
-        pb[(t * 2 + ci) * CH + co] = s[k];
+        __syncthreads();
+        for (int i = 0; i < GR; ++i)
+            for (int j = 16 * jg; j < 16 * jg + 16; ++j) {
+                const float d = bf2f(dz[(((size_t)n * H + y0 + i) * W + xb + j) * CH + co]);
+                sb += d;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    s[t][0] += xs[0][i + t / 3][j + t % 3] * d;
+                    s[t][1] += xs[1][i + t / 3][j + t % 3] * d;
+                }
+            }
     }
-    if (g == 0) pb[9 * 2 * CH + co] = sb;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        red[jg][(2 * t + 0) * CH + co] = s[t][0];
+        red[jg][(2 * t + 1) * CH + co] = s[t][1];
+    }
+    red[jg][9 * 2 * CH + co] = sb;
+    __syncthreads();
+    float *pb = part + (size_t)blockIdx.x * NO;
+    for (int i = threadIdx.x; i < NO; i += 256) pb[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
 }
 
-// out[i] = sum_b part[b * len + i] (FP64, in block order) for i < len.
-__global__ void k_reduce(const float *__restrict__ part, int nb, int len, float *__restrict__ out)
+// out[i] = sum_b part[b * len + i] (FP64, in block order) for i < len; the
+// loads of 32 partials are issued ahead of their in-order additions.
+__global__ __launch_bounds__(64) void k_reduce(const float *__restrict__ part, int nb, int len,
+                                               float *__restrict__ out)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += (double)part[(size_t)b * len + i];
-        out[i] = (float)s;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    int b = 0;
+    for (; b + 32 <= nb; b += 32) {
+        float v[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) v[k] = part[(size_t)(b + k) * len + i];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) s += (double)v[k];
     }
+    for (; b < nb; ++b) s += (double)part[(size_t)b * len + i];
+    out[i] = (float)s;
 }
 
 // BF16 copies of the layer weights: wf[l] = W_l (tap, co, ci) and wt[l] =
@@ -667,27 +889,41 @@ unsigned grid_for(long long n, int blk)
     return (unsigned)(g < 1 ? 1 : g);
 }
 
-// A 64 -> 64 layer in 128-pixel row segments (W is a multiple of 128): at
-// 256^2 that is 512 blocks, two per CU, all resident at once.
+// A 64 -> 64 layer: blocks of R rows x 64 pixels, R the largest multiple of
+// 8 dividing H that still gives one block per CU (256 at 256^2 x 16).
 void conv64(s2d_cnn *c, int B, const u16 *in, const u16 *wt, const float *bias, const u16 *zp,
             u16 *zo, u16 *ao, int mode)
 {
-    const int H = c->H, W = c->W;
-    hipLaunchKernelGGL(k_conv64<2>, dim3(B * H * (W / 128)), dim3(256), 0, c->stream, in, wt,
-                       bias, zp, zo, ao, H, W, mode);
+    const int H = c->H, W = c->W, segs = W / CV_SEG;
+    int R = 8;
+    while (H % (2 * R) == 0 && (long long)B * segs * (H / (2 * R)) >= 256) R *= 2;
+    hipLaunchKernelGGL(k_conv64, dim3(B * segs * (H / R)), dim3(CV_THREADS), 0, c->stream, in,
+                       wt, bias, zp, zo, ao, H, W, R, mode);
 }
 
+// Rows per k_dw64 block: the largest power of two dividing H that keeps at
+// least 512 blocks (two per CU), or 1.
+int dw_rows(int B, int H, int W)
+{
+    int R = 1;
+    while (H % (2 * R) == 0 && (long long)B * (W / DW_SEG) * (H / (2 * R)) >= 512) R *= 2;
+    return R;
+}
+long long dw_blocks(int B, int H, int W) { return (long long)B * (W / DW_SEG) * (H / dw_rows(B, H, W)); }
+
 // Forward of B samples into c->y (raw network output).
-int forward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, int ld)
+// keep_z: store the pre-activations z (the backward pass reads them).
+int forward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, int ld,
+            bool keep_z)
 {
     const int H = c->H, W = c->W;
-    const long long npx = (long long)B * H * W;
-    hipLaunchKernelGGL(k_conv_in, dim3(W / 32, B * H), dim3(256), 0, c->stream, x0, x1, sN,
-                       ld, c->in_scale[0], c->in_scale[1], c->P, c->z[0], c->a[0], B, H, W);
+    hipLaunchKernelGGL(k_conv_in, dim3(B * H * (W / C0_SEG)), dim3(256), 0, c->stream, x0, x1, sN,
+                       ld, c->in_scale[0], c->in_scale[1], c->P, keep_z ? c->z[0] : nullptr,
+                       c->a[0], B, H, W);
     CNN_TRY(launched(c, "k_conv_in"));
     for (int l = 1; l <= NMID; ++l) {
         conv64(c, B, c->a[l - 1], c->wf + (size_t)(l - 1) * W64, c->P + off_bl(l),
-               (const u16 *)nullptr, c->z[l], c->a[l], 0);
+               (const u16 *)nullptr, keep_z ? c->z[l] : nullptr, c->a[l], 0);
         CNN_TRY(launched(c, "k_conv64"));
     }
     hipLaunchKernelGGL(k_conv_out, dim3(W / 32, B * H), dim3(256), 0, c->stream, c->a[NMID],
@@ -704,37 +940,39 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
     hipLaunchKernelGGL(k_loss, dim3(c->lblocks), dim3(256), 0, c->stream, c->y, tg, tN, tld,
                        1.0f / c->out_scale, c->dy, c->lpart, B, H, W);
     CNN_TRY(launched(c, "k_loss"));
-    const int nrb = B * (H / OUT_ROWS);
+    const int nth = thin_blocks(B, H, W);
+    auto reduce = [&](int nb, int len, float *out) {
+        hipLaunchKernelGGL(k_reduce, dim3((len + 63) / 64), dim3(64), 0, c->stream, c->part, nb,
+                           len, out);
+        return launched(c, "k_reduce");
+    };
     // layer 5
-    hipLaunchKernelGGL(k_out_bwd_w, dim3(nrb), dim3(640), 0, c->stream, c->dy, c->a[NMID], H, W,
-                       c->part);
+    const int rth = thin_rows(B, H, W);
+    hipLaunchKernelGGL(k_out_bwd_w, dim3(nth), dim3(256), 0, c->stream, c->dy, c->a[NMID], H, W,
+                       rth, c->part);
     CNN_TRY(launched(c, "k_out_bwd_w"));
-    hipLaunchKernelGGL(k_reduce, dim3(grid_for(9 * CH + 1, 256)), dim3(256), 0, c->stream,
-                       c->part, nrb, 9 * CH + 1, c->G + OFF_W5);
-    CNN_TRY(launched(c, "k_reduce"));
-    hipLaunchKernelGGL(k_out_bwd_data, dim3(grid_for(npx, 256)), dim3(256), 0, c->stream, c->dy,
-                       c->P, c->z[NMID], c->dz[0], B, H, W);
+    CNN_TRY(reduce(nth, 9 * CH + 1, c->G + OFF_W5));
+    // a grid-stride loop over a few waves of blocks: each lane loads its 72 weights once
+    hipLaunchKernelGGL(k_out_bwd_data, dim3(std::min(grid_for(npx * 8, 256), 2048u)), dim3(256),
+                       0, c->stream,
+                       c->dy, c->P, c->z[NMID], c->dz[0], B, H, W);
     CNN_TRY(launched(c, "k_out_bwd_data"));
     int cur = 0;
-    const int ndw = B * (H / DW_ROWS);
+    const int rdw = dw_rows(B, H, W), ndw = (int)dw_blocks(B, H, W);
     for (int l = NMID; l >= 1; --l) {
-        hipLaunchKernelGGL(k_dw64, dim3(ndw), dim3(256), 0, c->stream, c->dz[cur], c->a[l - 1],
-                           H, W, c->part);
+        hipLaunchKernelGGL(k_dw64, dim3(ndw), dim3(DW_THREADS), 0, c->stream, c->dz[cur],
+                           c->a[l - 1], H, W, rdw, c->part);
         CNN_TRY(launched(c, "k_dw64"));
-        hipLaunchKernelGGL(k_reduce, dim3(grid_for(W64 + CH, 256)), dim3(256), 0, c->stream,
-                           c->part, ndw, W64 + CH, c->G + off_wl(l));
-        CNN_TRY(launched(c, "k_reduce"));
+        CNN_TRY(reduce(ndw, W64 + CH, c->G + off_wl(l)));
         conv64(c, B, c->dz[cur], c->wt + (size_t)(l - 1) * W64, (const float *)nullptr,
                c->z[l - 1], c->dz[1 - cur], (u16 *)nullptr, 1);
         CNN_TRY(launched(c, "k_conv64 (backward data)"));
         cur = 1 - cur;
     }
-    hipLaunchKernelGGL(k_in_bwd_w, dim3(nrb), dim3(192), 0, c->stream, x0, x1, sN, ld,
-                       c->in_scale[0], c->in_scale[1], c->dz[cur], H, W, c->part);
+    hipLaunchKernelGGL(k_in_bwd_w, dim3(nth), dim3(256), 0, c->stream, x0, x1, sN, ld,
+                       c->in_scale[0], c->in_scale[1], c->dz[cur], H, W, rth, c->part);
     CNN_TRY(launched(c, "k_in_bwd_w"));
-    hipLaunchKernelGGL(k_reduce, dim3(grid_for(9 * 2 * CH + CH, 256)), dim3(256), 0, c->stream,
-                       c->part, nrb, 9 * 2 * CH + CH, c->G + OFF_W0);
-    CNN_TRY(launched(c, "k_reduce"));
+    CNN_TRY(reduce(nth, 9 * 2 * CH + CH, c->G + OFF_W0));
     std::vector<double> lp((size_t)c->lblocks);
     CNN_HIP(c, hipMemcpyAsync(lp.data(), c->lpart, sizeof(double) * lp.size(),
                               hipMemcpyDeviceToHost, c->stream));
@@ -775,9 +1013,9 @@ int s2d_cnn_create(int H, int W, int batch_max, s2d_cnn **out)
 {
     if (!out) return cfail(nullptr, S2D_ERR_INVALID_ARG, "null output pointer");
     *out = nullptr;
-    if (H < 8 || W < TPX || W % TPX || H % 8 || batch_max < 1)
+    if (H < 8 || W < WSEG || W % WSEG || H % 8 || batch_max < 1)
         return cfail(nullptr, S2D_ERR_UNSUPPORTED,
-                     "grid %dx%d: W a multiple of %d, H a multiple of 8, batch >= 1", H, W, TPX);
+                     "grid %dx%d: W a multiple of %d, H a multiple of 8, batch >= 1", H, W, WSEG);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return cfail(nullptr, S2D_ERR_NO_DEVICE, "no GPU visible");
@@ -797,8 +1035,12 @@ int s2d_cnn_create(int H, int W, int batch_max, s2d_cnn **out)
     const size_t npx = (size_t)batch_max * H * W;
     const size_t act = npx * CH * sizeof(u16);
     c->lblocks = 1024;
-    const size_t nrb = (size_t)batch_max * (H / 8);
-    c->part_floats = nrb * (W64 + CH);
+    // per-block partial sums of the weight-gradient kernels, for any batch
+    size_t pf = 0;
+    for (int b = 1; b <= batch_max; ++b)
+        pf = std::max({pf, (size_t)dw_blocks(b, H, W) * (W64 + CH),
+                       (size_t)thin_blocks(b, H, W) * (9 * 2 * CH + CH)});
+    c->part_floats = pf;
     if (hipMalloc(&c->P, NPARAM * 4) || hipMalloc(&c->G, NPARAM * 4) ||
         hipMalloc(&c->M, NPARAM * 4) || hipMalloc(&c->V, NPARAM * 4) ||
         hipMalloc(&c->wf, (size_t)NMID * W64 * 2) || hipMalloc(&c->wt, (size_t)NMID * W64 * 2) ||
@@ -874,7 +1116,7 @@ int s2d_cnn_forward_on(s2d_cnn *c, void *stream, int B, const float *w, const fl
     CNN_HIP(c, hipSetDevice(c->device));
     hipStream_t own = c->stream;
     c->stream = static_cast<hipStream_t>(stream);
-    int r = forward(c, B, w, psi, sN, ld);
+    int r = forward(c, B, w, psi, sN, ld, false);
     if (r == S2D_OK) {
         hipLaunchKernelGGL(k_write_out, dim3(c->W / 128, B * c->H),
                            dim3(128), 0, c->stream, c->y, out, oN, old, c->out_scale, B, c->H,
@@ -891,7 +1133,7 @@ int s2d_cnn_forward(s2d_cnn *c, int B, const float *w, const float *psi, long lo
     CNN_TRY(check_batch(c, B));
     if (!w || !psi || !out) return cfail(c, S2D_ERR_INVALID_ARG, "null device pointer");
     CNN_HIP(c, hipSetDevice(c->device));
-    CNN_TRY(forward(c, B, w, psi, sN, ld));
+    CNN_TRY(forward(c, B, w, psi, sN, ld, false));
     hipLaunchKernelGGL(k_write_out, dim3(c->W / 128, B * c->H), dim3(128), 0,
                        c->stream, c->y, out, oN, old, c->out_scale, B, c->H, c->W, w, sN, ld,
                        c->clip);
@@ -906,7 +1148,7 @@ int s2d_cnn_gradient(s2d_cnn *c, int B, const float *w, const float *psi, long l
     CNN_TRY(check_batch(c, B));
     if (!w || !psi || !target) return cfail(c, S2D_ERR_INVALID_ARG, "null device pointer");
     CNN_HIP(c, hipSetDevice(c->device));
-    CNN_TRY(forward(c, B, w, psi, sN, ld));
+    CNN_TRY(forward(c, B, w, psi, sN, ld, true));
     CNN_TRY(backward(c, B, w, psi, sN, ld, target, tN, tld, loss));
     if (grad) {
         CNN_HIP(c, hipMemcpyAsync(grad, c->G, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
