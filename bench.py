@@ -323,6 +323,44 @@ def run_host_exchange(N: int, steps: int, nu: float, cfl: float):
             "host_bytes_per_step": 3 * w.nbytes}
 
 
+def run_cnn_train(steps: int, warmup: int, batch: int = 16, n: int = 256):
+    """The closure CNN's training step (forward, backward, Adam) at batch x n^2
+    on synthetic planes with the shipped weights; the FLOP count is that of
+    the six convolutions (forward, backward data except layer 0, weight
+    gradients)."""
+    import numpy as np
+    from spectral2d_amd.closure_run import SHIPPED
+    from spectral2d_amd.cnn import ClosureNet
+    rng = np.random.default_rng(7)
+    m = 2 * batch
+    w = rng.standard_normal((m, n, n)).astype(np.float32)
+    p = rng.standard_normal((m, n, n)).astype(np.float32)
+    t = (0.3 * rng.standard_normal((m, n, n))).astype(np.float32)
+    net = ClosureNet(n, n, batch)
+    with np.load(SHIPPED) as z:
+        net.set_params(z["params"].astype(np.float32))
+    net.set_scales(1.0, 1.0, 1.0)
+    net.set_dataset(w, p, t)
+    for _ in range(warmup):
+        net.train_batch(rng.choice(m, batch, replace=False), 1e-5)
+    idx = [rng.choice(m, batch, replace=False) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in idx:
+        loss = net.train_batch(i, 1e-5)  # synchronous (returns the loss)
+    el = time.perf_counter() - t0
+    net.close()
+    px = float(batch) * n * n
+    mac = px * 9 * (2 * 64 + 4 * 64 * 64 + 64)          # forward
+    flops = 2 * mac + 2 * px * 9 * (4 * 64 * 64 + 64)    # + backward data (layers 1..5)
+    flops += 2 * mac                                      # + weight gradients
+    ms = el / steps * 1e3
+    return {"metric": "closure CNN training step (forward, backward, Adam)",
+            "value": round(ms, 4), "unit": "ms per step", "higher_is_better": False,
+            "batch": batch, "grid": n, "steps": steps, "dtype": "bf16 (FP32 accumulation)",
+            "gflop_per_step": round(flops / 1e9, 1),
+            "tflops": round(flops / (ms * 1e-3) / 1e12, 1), "loss_last": round(loss, 5)}
+
+
 def run_closure(members: int):
     from spectral2d_amd.closure_run import eval_args, evaluate, load_net
     a = eval_args(eval_runs=members)
@@ -433,9 +471,10 @@ def main(argv=None):
                     help="parse two rocprofv3 --pmc output directories (FETCH_SIZE, "
                          "WRITE_SIZE runs made with --copy-calibration) into "
                          "profiles/traffic.json and exit")
-    ap.add_argument("--workload", choices=("all", "c1", "ensemble"), default="all",
+    ap.add_argument("--workload", choices=("all", "c1", "ensemble", "cnn"), default="all",
                     help="c1: the 1024^2 run only; ensemble: the ensemble only (its JSON "
-                         "object is printed alone); all: both on one line")
+                         "object is printed alone); cnn: the closure CNN's training step at "
+                         "256^2 x 16 alone; all: the bench line")
     ap.add_argument("--closure-members", type=int, default=4,
                     help="held-out members of the closure evaluation (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=500)
@@ -469,6 +508,9 @@ def main(argv=None):
 
     if args.workload == "ensemble":
         print(json.dumps(run_ensemble(args.ens_steps, args.ens_warmup, args.cfl)), flush=True)
+        return 0
+    if args.workload == "cnn":
+        print(json.dumps(run_cnn_train(min(args.steps, 200), min(args.warmup, 10))), flush=True)
         return 0
 
     N = args.n
