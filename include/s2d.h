@@ -412,6 +412,14 @@ int s2d_cnn_set_scales(s2d_cnn *net, float in0, float in1, float out);
 /* clip != 0: s2d_cnn_forward keeps tau only where omega * tau > 0 (the
  * closure term -tau then only removes enstrophy, pointwise); 0: raw */
 int s2d_cnn_set_clip(s2d_cnn *net, int clip);
+/* Launch width: blocks >= 1 makes every kernel use as few blocks as it can
+ * while keeping at least `blocks` (row-tiled kernels take more rows per
+ * block) and grid-stride kernels use at most `blocks`; 0 (the default)
+ * restores each kernel's own width (256 or 512 blocks). Results are the
+ * same to rounding (the forward bitwise); a width below the GPU's CU count
+ * only serves tests of the kernels' multi-row paths at small sizes.
+ * blocks <= 512 (the partial sums are sized for the default widths). */
+int s2d_cnn_set_launch_width(s2d_cnn *net, int blocks);
 /* tau (= out * network value) of B samples */
 int s2d_cnn_forward(s2d_cnn *net, int B, const float *w, const float *psi, long long stride,
                     int ld, float *tau, long long tau_stride, int tau_ld);

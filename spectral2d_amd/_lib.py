@@ -51,7 +51,8 @@ EXPORTS = (
     "s2d_cnn_last_error", "s2d_cnn_param_count", "s2d_cnn_create", "s2d_cnn_destroy",
     "s2d_cnn_set_params", "s2d_cnn_get_params", "s2d_cnn_set_scales", "s2d_cnn_forward",
     "s2d_cnn_gradient", "s2d_cnn_train_step", "s2d_cnn_set_dataset", "s2d_cnn_train_batch",
-    "s2d_cnn_predict", "s2d_cnn_set_clip", "s2d_cnn_forward_on", "s2d_cnn_grid",
+    "s2d_cnn_predict", "s2d_cnn_set_clip", "s2d_cnn_set_launch_width", "s2d_cnn_forward_on",
+    "s2d_cnn_grid",
     "s2d_closure_forcing", "s2d_closure_steps", "s2d_set_closure_guard",
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
@@ -201,6 +202,7 @@ def lib():
     L.s2d_cnn_train_batch.argtypes = [vp, C.c_int, ip, C.c_float, dp]
     L.s2d_cnn_predict.argtypes = [vp, ll, fp]
     L.s2d_cnn_set_clip.argtypes = [vp, C.c_int]
+    L.s2d_cnn_set_launch_width.argtypes = [vp, C.c_int]
     L.s2d_cnn_forward_on.argtypes = [vp, vp, C.c_int, vp, vp, ll, C.c_int, vp, ll, C.c_int]
     L.s2d_cnn_grid.argtypes = [vp, ip, ip]
     L.s2d_closure_forcing.argtypes = [vp, vp, vp]

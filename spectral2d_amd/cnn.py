@@ -63,6 +63,12 @@ class ClosureNet:
         """Keep only the enstrophy-sink part of tau (omega * tau > 0)."""
         self._c(L.lib().s2d_cnn_set_clip(self._h, int(bool(clip))))
 
+    def set_launch_width(self, blocks: int):
+        """Every kernel at the fewest blocks >= `blocks` (grid-stride ones at
+        most `blocks`); 0 restores the defaults. Same results; for tests of
+        the multi-row paths at small sizes."""
+        self._c(L.lib().s2d_cnn_set_launch_width(self._h, int(blocks)))
+
     def forward(self, B, w_ptr, psi_ptr, stride, ld, tau_ptr, tau_stride, tau_ld):
         self._c(L.lib().s2d_cnn_forward(self._h, int(B), C.c_void_p(w_ptr), C.c_void_p(psi_ptr),
                                         int(stride), int(ld), C.c_void_p(tau_ptr),

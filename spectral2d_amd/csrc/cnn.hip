@@ -659,15 +659,18 @@ __global__ __launch_bounds__(256) void k_out_bwd_data(const float *__restrict__ 
 // segment of R rows of one sample; thread (channel = lane, pixel group =
 // wave) walks 16 pixels of each row, accumulates in FP32, and the four
 // waves' sums are added in LDS. Per-block partial sums, reduced by k_reduce.
-// R: GR, doubled (dividing H) while at least 512 blocks remain.
+// R: GR, doubled (dividing H) while at least `width` blocks remain.
 constexpr int GR = 8, GSEG = 64;
-int thin_rows(int B, int H, int W)
+int thin_rows(int B, int H, int W, int width)
 {
     int R = GR;
-    while (H % (2 * R) == 0 && (long long)B * (H / (2 * R)) * (W / GSEG) >= 512) R *= 2;
+    while (H % (2 * R) == 0 && (long long)B * (H / (2 * R)) * (W / GSEG) >= width) R *= 2;
     return R;
 }
-int thin_blocks(int B, int H, int W) { return B * (H / thin_rows(B, H, W)) * (W / GSEG); }
+int thin_blocks(int B, int H, int W, int width)
+{
+    return B * (H / thin_rows(B, H, W, width)) * (W / GSEG);
+}
 
 // Layer 5: dW5[t][ci] = sum_p dy[p] a4[p + off_t][ci], db5 = sum_p dy[p];
 // partial layout [t][ci] then db5.
@@ -835,6 +838,7 @@ struct s2d_cnn {
     hipStream_t stream = nullptr;
     float in_scale[2] = {1.f, 1.f}, out_scale = 1.f;
     int clip = 0; // output mode of s2d_cnn_forward
+    int width = 0; // launch width of every kernel (0: each kernel's default)
     long long step = 0;
     // device buffers
     float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
@@ -893,6 +897,11 @@ int launched(s2d_cnn *c, const char *what)
         if (r_ != S2D_OK) return r_; \
     } while (0)
 
+// Launch widths (blocks a row-tiled launch keeps at least; the most blocks
+// of a grid-stride launch): each kernel's default, or the one
+// s2d_cnn_set_launch_width set for every kernel.
+int width_of(const s2d_cnn *c, int dflt) { return c->width > 0 ? c->width : dflt; }
+
 unsigned grid_for(long long n, int blk)
 {
     long long g = (n + blk - 1) / blk;
@@ -905,22 +914,25 @@ unsigned grid_for(long long n, int blk)
 void conv64(s2d_cnn *c, int B, const u16 *in, const u16 *wt, const float *bias, const u16 *zp,
             u16 *zo, u16 *ao, int mode)
 {
-    const int H = c->H, W = c->W, segs = W / CV_SEG;
+    const int H = c->H, W = c->W, segs = W / CV_SEG, width = width_of(c, 256);
     int R = 8;
-    while (H % (2 * R) == 0 && (long long)B * segs * (H / (2 * R)) >= 256) R *= 2;
+    while (H % (2 * R) == 0 && (long long)B * segs * (H / (2 * R)) >= width) R *= 2;
     hipLaunchKernelGGL(k_conv64, dim3(B * segs * (H / R)), dim3(CV_THREADS), 0, c->stream, in,
                        wt, bias, zp, zo, ao, H, W, R, mode);
 }
 
 // Rows per k_dw64 block: the largest power of two dividing H that keeps at
-// least 512 blocks (two per CU), or 1.
-int dw_rows(int B, int H, int W)
+// least `width` blocks (default 512, two per CU), or 1.
+int dw_rows(int B, int H, int W, int width)
 {
     int R = 1;
-    while (H % (2 * R) == 0 && (long long)B * (W / DW_SEG) * (H / (2 * R)) >= 512) R *= 2;
+    while (H % (2 * R) == 0 && (long long)B * (W / DW_SEG) * (H / (2 * R)) >= width) R *= 2;
     return R;
 }
-long long dw_blocks(int B, int H, int W) { return (long long)B * (W / DW_SEG) * (H / dw_rows(B, H, W)); }
+long long dw_blocks(int B, int H, int W, int width)
+{
+    return (long long)B * (W / DW_SEG) * (H / dw_rows(B, H, W, width));
+}
 
 // Forward of B samples into c->y (raw network output).
 // keep_z: store the pre-activations z (the backward pass reads them).
@@ -951,25 +963,26 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
     hipLaunchKernelGGL(k_loss, dim3(c->lblocks), dim3(256), 0, c->stream, c->y, tg, tN, tld,
                        1.0f / c->out_scale, c->dy, c->lpart, B, H, W);
     CNN_TRY(launched(c, "k_loss"));
-    const int nth = thin_blocks(B, H, W);
+    const int wth = width_of(c, 512), nth = thin_blocks(B, H, W, wth);
     auto reduce = [&](int nb, int len, float *out) {
         hipLaunchKernelGGL(k_reduce, dim3((len + 63) / 64), dim3(64), 0, c->stream, c->part, nb,
                            len, out);
         return launched(c, "k_reduce");
     };
     // layer 5
-    const int rth = thin_rows(B, H, W);
+    const int rth = thin_rows(B, H, W, wth);
     hipLaunchKernelGGL(k_out_bwd_w, dim3(nth), dim3(256), 0, c->stream, c->dy, c->a[NMID], H, W,
                        rth, c->part);
     CNN_TRY(launched(c, "k_out_bwd_w"));
     CNN_TRY(reduce(nth, 9 * CH + 1, c->G + OFF_W5));
     // a grid-stride loop over a few waves of blocks: each lane loads its 72 weights once
-    hipLaunchKernelGGL(k_out_bwd_data, dim3(std::min(grid_for(npx * 8, 256), 2048u)), dim3(256),
+    hipLaunchKernelGGL(k_out_bwd_data,
+                       dim3(std::min(grid_for(npx * 8, 256), (unsigned)width_of(c, 2048))), dim3(256),
                        0, c->stream,
                        c->dy, c->P, c->z[NMID], c->dz[0], B, H, W);
     CNN_TRY(launched(c, "k_out_bwd_data"));
     int cur = 0;
-    const int rdw = dw_rows(B, H, W), ndw = (int)dw_blocks(B, H, W);
+    const int rdw = dw_rows(B, H, W, width_of(c, 512)), ndw = (int)dw_blocks(B, H, W, width_of(c, 512));
     for (int l = NMID; l >= 1; --l) {
         hipLaunchKernelGGL(k_dw64, dim3(ndw), dim3(DW_THREADS), 0, c->stream, c->dz[cur],
                            c->a[l - 1], H, W, rdw, c->part);
@@ -1049,8 +1062,8 @@ int s2d_cnn_create(int H, int W, int batch_max, s2d_cnn **out)
     // per-block partial sums of the weight-gradient kernels, for any batch
     size_t pf = 0;
     for (int b = 1; b <= batch_max; ++b)
-        pf = std::max({pf, (size_t)dw_blocks(b, H, W) * (W64 + CH),
-                       (size_t)thin_blocks(b, H, W) * (9 * 2 * CH + CH)});
+        pf = std::max({pf, (size_t)dw_blocks(b, H, W, 512) * (W64 + CH),
+                       (size_t)thin_blocks(b, H, W, 512) * (9 * 2 * CH + CH)});
     c->part_floats = pf;
     if (hipMalloc(&c->P, NPARAM * 4) || hipMalloc(&c->G, NPARAM * 4) ||
         hipMalloc(&c->M, NPARAM * 4) || hipMalloc(&c->V, NPARAM * 4) ||
@@ -1109,6 +1122,15 @@ int s2d_cnn_grid(const s2d_cnn *c, int *H, int *W)
     if (!c || !H || !W) return cfail(nullptr, S2D_ERR_INVALID_ARG, "null argument");
     *H = c->H;
     *W = c->W;
+    return S2D_OK;
+}
+
+int s2d_cnn_set_launch_width(s2d_cnn *c, int blocks)
+{
+    if (!c) return cfail(nullptr, S2D_ERR_INVALID_ARG, "null network");
+    if (blocks < 0 || blocks > 512)
+        return cfail(c, S2D_ERR_INVALID_ARG, "launch width %d outside [0, 512]", blocks);
+    c->width = blocks;
     return S2D_OK;
 }
 

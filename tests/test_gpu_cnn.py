@@ -80,3 +80,39 @@ def test_training_lowers_the_loss():
     q = net.get_params()
     assert q.shape == p.shape and not np.array_equal(q, p)
     net.close()
+
+
+@pytest.mark.parametrize("H,W", [(64, 128), (32, 256)])
+def test_narrow_launches_take_the_multi_row_paths(H, W):
+    """Launch width 1: the 64->64 kernels walk all H rows per block (the
+    conv ring refilled H/8 - 1 times, the weight-gradient ring over every
+    row), the thin-layer gradients take every row chunk and the grid-stride
+    kernel loops over every pixel. The forward is bitwise the default
+    launches' (same accumulation order per output), the gradient equal to
+    rounding (other partial-sum grouping), both against the oracle."""
+    print(f"[cnn] launch width 1 vs default, H={H} W={W}", flush=True)
+    net, p, x, t, xw, xp, tg, stride, ld = _setup(B=2, H=H, W=W, seed=11)
+    out0 = torch.zeros_like(tg)
+    out1 = torch.zeros_like(tg)
+    net.forward(2, xw.data_ptr(), xp.data_ptr(), stride, ld, out0.data_ptr(), stride, ld)
+    loss0, g0 = net.gradient(2, xw.data_ptr(), xp.data_ptr(), stride, ld, tg.data_ptr(), stride, ld)
+    net.set_launch_width(1)
+    net.forward(2, xw.data_ptr(), xp.data_ptr(), stride, ld, out1.data_ptr(), stride, ld)
+    loss1, g1 = net.gradient(2, xw.data_ptr(), xp.data_ptr(), stride, ld, tg.data_ptr(), stride, ld)
+    torch.cuda.synchronize()
+    assert torch.equal(out0, out1)
+    assert loss1 == loss0
+    e = rel_l2(g1, g0)
+    print(f"  gradient, width 1 vs default: rel_l2 = {e:.2e}")
+    assert e < 1e-4
+    y_ref, _ = ocnn.forward(p.astype(np.float64), x, in_scale=(0.8, 1.25))
+    assert rel_l2(out1.cpu().numpy()[:, :, :W] / 0.5, y_ref) < 2e-3
+    _, g_ref, _ = ocnn.loss_and_grad(p.astype(np.float64), x, t, in_scale=(0.8, 1.25),
+                                     out_scale=0.5)
+    G, R = ocnn.unpack(g1.astype(np.float64)), ocnn.unpack(g_ref)
+    for k in R:
+        assert rel_l2(G[k], R[k]) < 2e-2, k
+    with pytest.raises(Exception):
+        net.set_launch_width(513)
+    net.set_launch_width(0)
+    net.close()
