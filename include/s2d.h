@@ -202,6 +202,13 @@ int s2d_spectrum_uv(s2d_ctx *ctx, int u, int v, double nu, double *ek, int *cnt,
 /* Simulation time / step count held by the context (advanced by the step
  * calls); s2d_set_time resets them. */
 int s2d_set_time(s2d_ctx *ctx, double time, long long nadv);
+/* End time of the fused steps (s2d_rk3_steps, s2d_closure_steps): t_end > 0
+ * clamps every step's dt (fixed or CFL) so that the run lands on t_end; once
+ * there, further steps have dt = 0 and keep the state exactly (time and the
+ * step count stay). t_end <= 0 removes the limit. No reference counterpart:
+ * the reference's driver clamps its last step on the host
+ * (src/Spectral2D.F90 DESIRED_END_TIME), which a caller can still do. */
+int s2d_set_end_time(s2d_ctx *ctx, double t_end);
 /* time, step count and dt of the last step (any pointer may be NULL) */
 int s2d_get_time(s2d_ctx *ctx, double *time, long long *nadv, double *dt);
 
@@ -336,6 +343,10 @@ int s2d_ensemble_size(const s2d_ensemble *ens, int *nruns, int *nclasses);
 int s2d_ensemble_run_grid(s2d_ensemble *ens, int run, s2d_grid *g);
 /* nu[nruns] */
 int s2d_ensemble_set_viscosity(s2d_ensemble *ens, const double *nu);
+/* s2d_set_end_time of every run: t_end[nruns] (<= 0: none), or NULL to
+ * clear them all; each run lands on its own end time and then keeps its
+ * state while the others go on. */
+int s2d_ensemble_set_end_time(s2d_ensemble *ens, const double *t_end);
 /* s2d_set_turbulence of every run: seed[nruns], k0[nruns], ke[nruns]. */
 int s2d_ensemble_set_turbulence(s2d_ensemble *ens, const unsigned long long *seed,
                                 const double *k0, const double *ke);
@@ -349,6 +360,18 @@ int s2d_ensemble_get_field(s2d_ensemble *ens, int run, int field, double *buf);
  * step is per run). flags: 0 or S2D_ENS_GRAPH. Every run is checked for
  * NaN/Inf at the end. */
 int s2d_ensemble_steps(s2d_ensemble *ens, int nsteps, double dt, double cfl, int flags);
+/* s2d_closure_steps for every run of an ensemble of one size: nsteps
+ * graph-replayed steps with the closure forcing of each run, refreshed every
+ * `every` steps (the period runs on across calls) by one network call at
+ * B = runs (the network's batch size must allow it); guard != 0: the shell
+ * guard (s2d_set_closure_guard). Each run's forcing, and so its trajectory,
+ * is bitwise the one s2d_closure_steps computes for that run alone. grid:
+ * an FP32 S2D_NO_DEALIASING context of the runs' size; net: a network of
+ * that grid. The forcing stays on for later s2d_ensemble_steps until
+ * s2d_ensemble_clear_forcing. No reference counterpart. */
+int s2d_ensemble_closure_steps(s2d_ensemble *ens, s2d_ctx *grid, s2d_cnn *net, int nsteps,
+                               double dt, double cfl, int every, int guard);
+int s2d_ensemble_clear_forcing(s2d_ensemble *ens);
 /* s2d_spectrum of one run. */
 int s2d_ensemble_spectrum(s2d_ensemble *ens, int run, double nu, double *ek, int *cnt,
                           double *tke, double *eps, double *ediss);

@@ -43,6 +43,7 @@ EXPORTS = (
     "s2d_compute_psi", "s2d_compute_velocity", "s2d_compute_vorticity",
     "s2d_compute_rhs", "s2d_integrate_one_step", "s2d_rk3_step", "s2d_rk3_steps",
     "s2d_cfl_dt", "s2d_diagnostics", "s2d_diagnostics_noalloc", "s2d_set_time",
+    "s2d_set_end_time",
     "s2d_get_time", "s2d_profile_begin",
     "s2d_profile_every", "s2d_profile_end", "s2d_set_turbulence", "s2d_set_taylor_green",
     "s2d_spectrum_bins", "s2d_spectrum", "s2d_spectrum_uv", "s2d_ensemble_spectrum",
@@ -56,6 +57,7 @@ EXPORTS = (
     "s2d_closure_forcing", "s2d_closure_steps", "s2d_set_closure_guard",
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
+    "s2d_ensemble_set_end_time", "s2d_ensemble_closure_steps", "s2d_ensemble_clear_forcing",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
     "s2d_ensemble_steps", "s2d_ensemble_diagnostics", "s2d_ensemble_set_schedule",
     "s2d_ensemble_profile_begin",
@@ -153,6 +155,7 @@ def lib():
     L.s2d_cfl_dt.argtypes = [vp, C.c_double, dp]
     L.s2d_diagnostics.argtypes = [vp, P(Diag)]
     L.s2d_set_time.argtypes = [vp, C.c_double, C.c_longlong]
+    L.s2d_set_end_time.argtypes = [vp, C.c_double]
     L.s2d_get_time.argtypes = [vp, dp, P(C.c_longlong), dp]
     L.s2d_profile_begin.argtypes = [vp, C.c_int]
     L.s2d_profile_every.argtypes = [vp, C.c_int]
@@ -177,6 +180,10 @@ def lib():
     L.s2d_ensemble_size.argtypes = [vp, ip, ip]
     L.s2d_ensemble_run_grid.argtypes = [vp, C.c_int, P(Grid)]
     L.s2d_ensemble_set_viscosity.argtypes = [vp, dp]
+    L.s2d_ensemble_set_end_time.argtypes = [vp, dp]
+    L.s2d_ensemble_closure_steps.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_double,
+                                             C.c_int, C.c_int]
+    L.s2d_ensemble_clear_forcing.argtypes = [vp]
     L.s2d_ensemble_set_turbulence.argtypes = [vp, P(C.c_ulonglong), dp, dp]
     L.s2d_ensemble_set_vorticity.argtypes = [vp, C.c_int, dp]
     L.s2d_ensemble_get_field.argtypes = [vp, C.c_int, C.c_int, dp]

@@ -67,6 +67,7 @@ __global__ void k_passA(const cpx<T> *__restrict__ Wsrc, cpx<T> *__restrict__ Gu
     const int kx = blockIdx.x;
     if (reset_max && kx == 0 && threadIdx.x == 0) {
         sc->vmax_bits = 0ull;
+        sc->t_start = sc->time;
         sc->bad = 0u;
     }
     const cpx<T> *src = Wsrc + (size_t)kx * d.KP;
@@ -228,13 +229,22 @@ struct StageArgs {
     double inv_norm; // 1 / norm
 };
 
+// The step's dt: the CFL dt of this step's velocity maximum (or the fixed
+// dt), clamped so that the step ends at t_end when one is set (t_start is
+// the time at the step's start, fixed for its three stages); 0 once the run
+// has reached t_end (the stage update then keeps the state as it is).
 __device__ __forceinline__ double stage_dt(const StageArgs &a, const DevScalars *sc)
 {
+    double d = a.dt;
     if (a.cfl > 0.0) {
         const double m = __longlong_as_double((long long)sc->vmax_bits);
-        if (m > 0.0 && m < 1.0e300) return a.cfl * a.dx / m;
+        if (m > 0.0 && m < 1.0e300) d = a.cfl * a.dx / m;
     }
-    return a.dt;
+    if (sc->t_end > 0.0) {
+        const double rem = sc->t_end - sc->t_start;
+        d = rem > 1.0e-12 * sc->t_end ? (d < rem ? d : rem) : 0.0;
+    }
+    return d;
 }
 
 template <typename T, int E>
@@ -278,7 +288,9 @@ __global__ void k_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict
             cpx<T> ws = Wst[col + c];
             cpx<T> dW = mk<T>((-du.re - dv.re) - sn * ws.re, (-du.im - dv.im) - sn * ws.im);
             cpx<T> w0 = Wn[col + c];
-            if (a.stage == 1) {
+            if (dtd == 0.0) {
+                out = w0; // a run at its end time keeps its state
+            } else if (a.stage == 1) {
                 out = mk<T>(w0.re + dt * dW.re, w0.im + dt * dW.im);
             } else if (a.stage == 2) {
                 const T cc = (T)0.25 * dt;
@@ -295,7 +307,7 @@ __global__ void k_passC(const cpx<T> *__restrict__ Huw, const cpx<T> *__restrict
     if (a.stage == 3 && kx == 0 && threadIdx.x == 0) {
         sc->dt = dtd;
         sc->time = sc->time + dtd;
-        sc->nadv = sc->nadv + 1;
+        if (!(dtd == 0.0 && sc->t_end > 0.0)) sc->nadv = sc->nadv + 1;
     }
 }
 

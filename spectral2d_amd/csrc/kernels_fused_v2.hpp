@@ -284,6 +284,7 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     const bool active = kx < d.K;
     if (reset_max && f == 0 && kx0 == 0 && threadIdx.x == 0) {
         sc[run].vmax_bits = 0ull;
+        sc[run].t_start = sc[run].time;
         sc[run].bad = 0u;
     }
     const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
@@ -387,6 +388,7 @@ __device__ __forceinline__ void passA_fused_body(int blk, int nblk, cpx<T> *lds,
     const bool active = kx < d.K;
     if (reset_max && kx0 == 0 && threadIdx.x == 0) {
         sc[run].vmax_bits = 0ull;
+        sc[run].t_start = sc[run].time;
         sc[run].bad = 0u;
     }
     const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
@@ -864,7 +866,8 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP, V>
             const cpx<T> wsv = op.ws[i], w0v = op.w0[i];
             cpx<T> dW = mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
             if (forced) dW = mk<T>(dW.re - op.fv[i].re, dW.im - op.fv[i].im); // closure / forcing
-            cpx<T> out = rk3_update<T>(a.stage, dt, w0v, wsv, dW);
+            // a run at its end time (dt = 0) keeps its state
+            cpx<T> out = dtd == 0.0 ? w0v : rk3_update<T>(a.stage, dt, w0v, wsv, dW);
             out = sel(keep, out, mk<T>((T)0, (T)0));
             if (active && op.row[i]) Wout[col + op.c[i]] = out;
         }
@@ -924,7 +927,7 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     if (HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0) {
         sc[run].dt = dtd;
         sc[run].time = sc[run].time + dtd;
-        sc[run].nadv = sc[run].nadv + 1;
+        if (!(dtd == 0.0 && sc[run].t_end > 0.0)) sc[run].nadv = sc[run].nadv + 1;
     }
 }
 
@@ -967,7 +970,8 @@ __device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T
         cpx<T> dW = mk<T>((-du.re) - sn * wsv.re, (-du.im) - sn * wsv.im);
         const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
         if (Fc) dW = mk<T>(dW.re - Fc[col].re, dW.im - Fc[col].im);
-        Wout[col] = rk3_update<T>(a.stage, (T)stage_dt(a, sc + run), w0v, wsv, dW);
+        const double dtd = stage_dt(a, sc + run);
+        Wout[col] = dtd == 0.0 ? w0v : rk3_update<T>(a.stage, (T)dtd, w0v, wsv, dW);
     }
     for (int c = 1 + threadIdx.x; c <= d.N; c += NT) Wout[col + c] = mk<T>((T)0, (T)0);
 }

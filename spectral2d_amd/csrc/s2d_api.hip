@@ -1020,53 +1020,69 @@ int check_closure(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net)
     return S2D_OK;
 }
 
-// The closure forcing sequence on c's stream, no waits: (optionally the
-// compact state into W / PSI,) the cutoff sampling onto g, two c2r, the
-// network, r2c, the restriction back, the pack into the forcing buffer.
-int enqueue_closure_forcing(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, bool unpack)
+// The closure forcing sequence on c's stream, no waits, in two halves around
+// the network call. closure_inputs: (optionally run `run`'s compact state
+// into W / PSI,) the cutoff sampling onto g, two c2r (the network's inputs in
+// g's W and PSI). closure_output: g's TMP (the network's output) by r2c, the
+// restriction back, the pack into run `run`'s forcing buffer, the shell guard.
+int closure_restrict(s2d_ctx *c, s2d_ctx *src, int sf, s2d_ctx *dst, int df)
 {
-    if (unpack) S2D_TRY(op_unpack<float>(c, 0));
+    const long long kc = c->g.nx / 2;
+    const size_t n = nc_of(dst);
+    hipLaunchKernelGGL((k_restrict<float, float>), dim3(grid_blocks(n, 256)), dim3(256), 0,
+                       c->stream, F<float>(src, sf), src->g.cISize, src->g.nyG,
+                       F<float>(dst, df), dst->g.cISize, dst->g.nyG, kc * kc);
+    return hipGetLastError() == hipSuccess ? S2D_OK
+                                           : fail(c, S2D_ERR_HIP, "launch of k_restrict failed");
+}
+
+int closure_inputs(s2d_ctx *c, s2d_ctx *g, int run, bool unpack)
+{
+    if (unpack) S2D_TRY(op_unpack<float>(c, run));
     hipStream_t gs = g->stream;
     g->stream = c->stream; // the whole sequence on one stream
-    const long long kc = c->g.nx / 2;
-    auto restrict_ = [&](s2d_ctx *src, int sf, s2d_ctx *dst, int df) {
-        const size_t n = nc_of(dst);
-        hipLaunchKernelGGL((k_restrict<float, float>), dim3(grid_blocks(n, 256)), dim3(256), 0,
-                           c->stream, F<float>(src, sf), src->g.cISize, src->g.nyG,
-                           F<float>(dst, df), dst->g.cISize, dst->g.nyG, kc * kc);
-        return hipGetLastError() == hipSuccess ? S2D_OK
-                                               : fail(c, S2D_ERR_HIP, "launch of k_restrict failed");
-    };
-    int r = restrict_(c, S2D_FIELD_W, g, S2D_FIELD_W);
-    if (!r) r = restrict_(c, S2D_FIELD_PSI, g, S2D_FIELD_PSI);
+    int r = closure_restrict(c, c, S2D_FIELD_W, g, S2D_FIELD_W);
+    if (!r) r = closure_restrict(c, c, S2D_FIELD_PSI, g, S2D_FIELD_PSI);
     if (!r) r = op_c2r<float>(g, S2D_FIELD_W);
     if (!r) r = op_c2r<float>(g, S2D_FIELD_PSI);
-    if (!r) {
-        const int ld = g->g.rISize; // the real layout of the grid context
-        if (s2d_cnn_forward_on(net, c->stream, 1, static_cast<const float *>(g->field[S2D_FIELD_W]),
-                               static_cast<const float *>(g->field[S2D_FIELD_PSI]), 0, ld,
-                               static_cast<float *>(g->field[S2D_FIELD_TMP]), 0, ld) != S2D_OK)
-            r = fail(c, S2D_ERR_HIP, "closure network: %s", s2d_cnn_last_error(net));
-    }
-    if (!r) r = op_r2c<float>(g, S2D_FIELD_TMP);
-    if (!r) r = restrict_(g, S2D_FIELD_TMP, c, S2D_FIELD_TMP);
+    g->stream = gs;
+    return r;
+}
+
+int closure_output(s2d_ctx *c, s2d_ctx *g, int run)
+{
+    hipStream_t gs = g->stream;
+    g->stream = c->stream;
+    int r = op_r2c<float>(g, S2D_FIELD_TMP);
+    if (!r) r = closure_restrict(c, g, S2D_FIELD_TMP, c, S2D_FIELD_TMP);
     g->stream = gs;
     S2D_TRY(r);
     // the kept modes of TMP into the forcing buffer (as s2d_set_forcing)
     void *keep = c->Wn, *wslot = c->field[S2D_FIELD_W];
     c->Wn = c->forcing;
     c->field[S2D_FIELD_W] = c->field[S2D_FIELD_TMP];
-    r = op_pack<float>(c, 0);
+    r = op_pack<float>(c, run);
     c->field[S2D_FIELD_W] = wslot;
     c->Wn = keep;
     if (!r && c->closure_guard) {
         const int nshell = (int)std::floor(std::sqrt(2.0) * (c->N / 2)) + 2;
         hipLaunchKernelGGL(k_shell_guard<float>, dim3(nshell), dim3(256), 0, c->stream,
-                           (cpx<float> *)c->forcing, (const cpx<float> *)c->Wn, c->N, c->K,
-                           c->KP);
+                           (cpx<float> *)c->forcing + run * c->sW,
+                           (const cpx<float> *)c->Wn + run * c->sW, c->N, c->K, c->KP);
         if (hipGetLastError() != hipSuccess) r = fail(c, S2D_ERR_HIP, "launch of k_shell_guard failed");
     }
     return r;
+}
+
+int enqueue_closure_forcing(s2d_ctx *c, s2d_ctx *g, s2d_cnn *net, bool unpack)
+{
+    S2D_TRY(closure_inputs(c, g, 0, unpack));
+    const int ld = g->g.rISize; // the real layout of the grid context
+    if (s2d_cnn_forward_on(net, c->stream, 1, static_cast<const float *>(g->field[S2D_FIELD_W]),
+                           static_cast<const float *>(g->field[S2D_FIELD_PSI]), 0, ld,
+                           static_cast<float *>(g->field[S2D_FIELD_TMP]), 0, ld) != S2D_OK)
+        return fail(c, S2D_ERR_HIP, "closure network: %s", s2d_cnn_last_error(net));
+    return closure_output(c, g, 0);
 }
 
 void drop_closure_graphs(s2d_ctx *c)
@@ -1695,7 +1711,8 @@ int s2d_rk3_steps(s2d_ctx *c, int nsteps, double nu, double dt, double cfl)
         return fail(c, S2D_ERR_NONFINITE,
                     "non-finite values in the state (max |u|+|v| at the last step start = %g)",
                     vmax);
-    if (cfl > 0.0 && !(sc.dt > 0.0 && sc.dt < 1.0e300))
+    // (dt = 0 is a run that has reached its end time)
+    if (cfl > 0.0 && !(sc.dt > 0.0 && sc.dt < 1.0e300) && !(sc.t_end > 0.0 && sc.dt == 0.0))
         return fail(c, S2D_ERR_NONFINITE, "CFL time step collapsed (dt = %g)", sc.dt);
     return S2D_OK;
 }
@@ -1945,6 +1962,21 @@ int s2d_get_time(s2d_ctx *c, double *time, long long *nadv, double *dt)
     if (nadv) *nadv = sc.nadv;
     if (dt) *dt = sc.dt;
     return S2D_OK;
+}
+
+int s2d_set_end_time(s2d_ctx *c, double t_end)
+{
+    S2D_TRY(check_ctx(c));
+    if (!(t_end < 1.0e300)) return fail(c, S2D_ERR_INVALID_ARG, "invalid end time");
+    S2D_TRY(set_device(c));
+    std::vector<DevScalars> sc((size_t)c->nruns);
+    S2D_HIP(c, hipMemcpyAsync(sc.data(), c->d_sc, sizeof(DevScalars) * sc.size(),
+                              hipMemcpyDeviceToHost, c->stream));
+    S2D_TRY(sync(c));
+    for (auto &s : sc) s.t_end = t_end > 0.0 ? t_end : 0.0;
+    S2D_HIP(c, hipMemcpyAsync(c->d_sc, sc.data(), sizeof(DevScalars) * sc.size(),
+                              hipMemcpyHostToDevice, c->stream));
+    return sync(c);
 }
 
 int s2d_set_time(s2d_ctx *c, double time, long long nadv)

@@ -34,7 +34,9 @@ struct DevScalars {
     long long nadv;               // accumulated steps
     unsigned int bad;             // nonzero: a non-finite value was seen
     unsigned int pad32;
-    double pad[3];
+    double t_end;                 // > 0: every step's dt is clamped to end here
+    double t_start;               // time at the start of the current step
+    double pad;
 };
 
 } // namespace s2d

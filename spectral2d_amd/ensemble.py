@@ -68,6 +68,27 @@ class Ensemble:
         nu = np.ascontiguousarray(np.broadcast_to(np.asarray(nu, np.float64), (self.nruns,)))
         self._c(L.lib().s2d_ensemble_set_viscosity(self._h, self._d(nu)))
 
+    def set_end_time(self, t_end=None):
+        """Per-run end times (> 0; each run lands on its own and then keeps its
+        state); None clears them."""
+        if t_end is None:
+            self._c(L.lib().s2d_ensemble_set_end_time(self._h, None))
+            return
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(t_end, np.float64), (self.nruns,)))
+        self._c(L.lib().s2d_ensemble_set_end_time(self._h, self._d(t)))
+
+    def closure_steps(self, grid, net, nsteps: int, dt: float = 0.0, cfl: float = 0.5,
+                      every: int = 8, guard: bool = True):
+        """s2d_ensemble_closure_steps: the closure forcing of every run from one
+        network call (grid: an FP32 unpadded Spectral2D of the runs' size;
+        net: a ClosureNet of that grid with batch >= runs)."""
+        self._c(L.lib().s2d_ensemble_closure_steps(self._h, grid._h, net._h, int(nsteps),
+                                                   float(dt), float(cfl), int(every),
+                                                   int(bool(guard))))
+
+    def clear_forcing(self):
+        self._c(L.lib().s2d_ensemble_clear_forcing(self._h))
+
     def set_turbulence(self, seed, k0, ke=0.5):
         """Spectrum/phase recipe of ``Spectral2D.SetTurbulence`` for every run."""
         seed = np.ascontiguousarray(np.broadcast_to(np.asarray(seed, np.uint64), (self.nruns,)))

@@ -309,6 +309,10 @@ class Spectral2D:
     def set_time(self, time: float = 0.0, nadv: int = 0):
         self._c(L.lib().s2d_set_time(self._h, float(time), int(nadv)))
 
+    def set_end_time(self, t_end: float = 0.0):
+        """Fused steps land on t_end (> 0) and then keep the state; 0: no end."""
+        self._c(L.lib().s2d_set_end_time(self._h, float(t_end)))
+
 
 __all__ = ["GetGridSize", "Wavenumbers", "Spectral2D", "S2DError", "FIELDS", "FP64", "FP32",
            "DEALIAS_3_2_RULE", "FIELD_W", "FIELD_PSI", "FIELD_U", "FIELD_V", "FIELD_WS",

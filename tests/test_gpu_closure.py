@@ -95,3 +95,46 @@ def test_captured_closure_steps_equal_eager_steps():
     for r in (a, b):
         r.close()
     net.close()
+
+
+def test_ensemble_closure_equals_runs_alone():
+    """s2d_ensemble_closure_steps (one network call at B = 3 per refresh,
+    refreshed every 4 steps, shell guard on, CFL steps with per-run end
+    times) gives each member bitwise the trajectory of s2d_closure_steps for
+    that run alone, across two calls (the period runs on)."""
+    from spectral2d_amd.ensemble import Ensemble
+    N, M, every, nu = 128, 3, 4, 1e-4
+    t_end = [0.02, 0.0, 0.015]
+    print(f"[closure] ensemble of {M} x {N}^2 vs runs alone, every={every}", flush=True)
+    p = ocnn.init_params(4).astype(np.float32)
+    netb = ClosureNet(N, N, M)
+    net1 = ClosureNet(N, N, 1)
+    for n in (netb, net1):
+        n.set_params(p)
+        n.set_scales(0.2, 5.0, 0.05)
+    e = Ensemble([N] * M, FP32)
+    e.set_viscosity(nu)
+    e.set_end_time(t_end)
+    from spectral2d_amd._lib import NO_DEALIASING
+    grid = Spectral2D(N, dealias=NO_DEALIASING, precision=FP32)
+    alone = []
+    for r in range(M):
+        a = ClosedRun(N, net1, every=every, guard=True)
+        a.sim.SetTurbulence(20 + r, 8.0, 0.5)
+        a.sim.set_time(0.0, 0)
+        a.sim.set_end_time(t_end[r])
+        e.set_vorticity(r, a.sim.get_field("w"))
+        alone.append(a)
+    for n in (7, 6):
+        e.closure_steps(grid, netb, n, cfl=0.5, every=every, guard=True)
+        for a in alone:
+            a.steps(n, nu, cfl=0.5)
+    d = e.diagnostics()
+    for r, a in enumerate(alone):
+        assert np.array_equal(e.get_field(r, "w"), a.sim.get_field("w")), r
+        assert d[r]["time"] == a.sim.get_time()[0]
+        a.close()
+    e.close()
+    grid.close()
+    netb.close()
+    net1.close()
