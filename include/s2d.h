@@ -372,6 +372,9 @@ int s2d_ensemble_steps(s2d_ensemble *ens, int nsteps, double dt, double cfl, int
 int s2d_ensemble_closure_steps(s2d_ensemble *ens, s2d_ctx *grid, s2d_cnn *net, int nsteps,
                                double dt, double cfl, int every, int guard);
 int s2d_ensemble_clear_forcing(s2d_ensemble *ens);
+/* time[nruns], nadv[nruns] (either may be NULL): each run's simulation time
+ * and step count (s2d_get_time of each run, one read-back). */
+int s2d_ensemble_get_time(s2d_ensemble *ens, double *time, long long *nadv);
 /* s2d_spectrum of one run. */
 int s2d_ensemble_spectrum(s2d_ensemble *ens, int run, double nu, double *ek, int *cnt,
                           double *tke, double *eps, double *ediss);

@@ -58,6 +58,7 @@ EXPORTS = (
     "s2d_ensemble_last_error", "s2d_ensemble_create", "s2d_ensemble_destroy",
     "s2d_ensemble_size", "s2d_ensemble_run_grid", "s2d_ensemble_set_viscosity",
     "s2d_ensemble_set_end_time", "s2d_ensemble_closure_steps", "s2d_ensemble_clear_forcing",
+    "s2d_ensemble_get_time",
     "s2d_ensemble_set_turbulence", "s2d_ensemble_set_vorticity", "s2d_ensemble_get_field",
     "s2d_ensemble_steps", "s2d_ensemble_diagnostics", "s2d_ensemble_set_schedule",
     "s2d_ensemble_profile_begin",
@@ -184,6 +185,7 @@ def lib():
     L.s2d_ensemble_closure_steps.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_double,
                                              C.c_int, C.c_int]
     L.s2d_ensemble_clear_forcing.argtypes = [vp]
+    L.s2d_ensemble_get_time.argtypes = [vp, dp, C.POINTER(C.c_longlong)]
     L.s2d_ensemble_set_turbulence.argtypes = [vp, P(C.c_ulonglong), dp, dp]
     L.s2d_ensemble_set_vorticity.argtypes = [vp, C.c_int, dp]
     L.s2d_ensemble_get_field.argtypes = [vp, C.c_int, C.c_int, dp]

@@ -113,7 +113,7 @@ def load_net(path: str = SHIPPED, nc: int = 256):
 def eval_args(**kw):
     """The evaluation settings of main() (C4 defaults), overridable."""
     base = dict(nf=2048, nc=256, nu=2e-5, k0=12.0, cfl=0.5, eval_runs=64, eval_seed0=3000,
-                t0=0.5, t1=1.0, closure_every=8, guard=1)
+                t0=0.5, t1=1.0, closure_every=8, guard=1, batched=1)
     base.update(kw)
     return argparse.Namespace(**base)
 
@@ -123,7 +123,113 @@ def evaluate(a, net, log=log, extra=None):
     256^2 run, a closure run with `net` and, for each (name, network) of
     `extra`, a run with that network (same guard and refresh period), all
     to t1 while the DNS continues; errors of each kind's ensemble-mean
-    enstrophy spectrum against the filtered DNS."""
+    enstrophy spectrum against the filtered DNS. a.batched (the default):
+    the coarse runs of each kind go as one ensemble (one network call per
+    refresh for all members, every member landing on t1 by its own end
+    time); otherwise member by member."""
+    if getattr(a, "batched", 1):
+        return _evaluate_batched(a, net, log, extra)
+    return _evaluate_per_member(a, net, log, extra)
+
+
+def _batched_net(net, M):
+    """A ClosureNet of batch M with `net`'s parameters and scales."""
+    from .cnn import ClosureNet
+    b = ClosureNet(net.H, net.W, M)
+    b.set_params(net.get_params())
+    b.set_scales(*net.scales)
+    b.set_clip(net.clip)
+    return b
+
+
+def _evaluate_batched(a, net, log=log, extra=None):
+    from ._lib import NO_DEALIASING
+    from .closure import FilteredSampler, Timer, advance_to, enstrophy_spectrum
+    from .closure import spectrum_error
+    from .ensemble import Ensemble
+    from .spectral import FP32, Spectral2D
+    extra = dict(extra or {})
+    kmax, M, span = a.nc // 2, a.eval_runs, a.t1 - a.t0
+    t_dns = Timer()
+    steps = {"dns": 0}
+    z_dns, w0 = [], []
+    coarse = Spectral2D(a.nc, precision=FP32)
+    for m in range(M):
+        s = make_dns(a.nf, a.eval_seed0 + m, a.k0, a.nu)
+        advance_to(s, a.t0, a.nu, a.cfl)
+        s.restrict_to("w", coarse, "w", kmax)
+        w0.append(coarse.get_field("w"))
+        with t_dns:
+            steps["dns"] += advance_to(s, a.t1, a.nu, a.cfl)
+        fs = FilteredSampler(s, a.nc)
+        fs.filtered_state(fs.coarse)
+        fs.coarse.ComputePsi("w", "psi")
+        z_dns.append(enstrophy_spectrum(fs.coarse))
+        fs.close()
+        s.close()
+        log(f"eval member {m}: DNS reached t1 ({steps['dns']} steps so far)")
+    coarse.close()
+    grid = Spectral2D(a.nc, dealias=NO_DEALIASING, precision=FP32)
+    nets = {"plain": None, "closure": net, **extra}
+    Z, times, unstable = {}, {}, {}
+    for kind, kn in nets.items():
+        e = Ensemble([a.nc] * M, FP32)
+        e.set_viscosity(a.nu)
+        for m in range(M):
+            e.set_vorticity(m, w0[m])
+        e.set_end_time(span)  # members start at 0 and land on t1 - t0
+        bn = _batched_net(kn, M) if kn is not None else None
+        t, n = Timer(), 0
+        try:
+            with t:
+                while min(e.get_time()[0]) < span * (1 - 1e-12):
+                    if n > 20000:
+                        raise RuntimeError("end time not reached in 20000 steps")
+                    if bn is None:
+                        e.steps(50, cfl=a.cfl, graph=True)
+                    else:
+                        e.closure_steps(grid, bn, 48, cfl=a.cfl, every=a.closure_every,
+                                        guard=bool(a.guard))
+                    n += 48 if bn is not None else 50
+            Z[kind] = [enstrophy_spectrum_of(e, m) for m in range(M)]
+            steps[kind] = int(sum(e.get_time()[1]))
+        except RuntimeError as ex:
+            log(f"  {kind} ensemble unstable: {ex}")
+            unstable[kind] = M
+        times[kind] = t.t
+        log(f"{kind}: {M} members to t1 in {t.t:.3f} s")
+        if bn is not None:
+            bn.close()
+        e.close()
+    grid.close()
+    zd = np.mean(z_dns, 0)
+    if "closure" not in Z:
+        return {"members": M, "unstable": unstable, "steps": steps}
+    err = {k: (spectrum_error(np.mean(Z[k], 0), zd, kmax) if k in Z else float("inf"))
+           for k in nets}
+    zp, zc = np.mean(Z["plain"], 0), np.mean(Z["closure"], 0)
+    return {
+        **{f"err_{k}": err[k] for k in extra},
+        "unstable": unstable,
+        "members": M, "batched": True,
+        "t0": a.t0, "t1": a.t1,
+        "err_plain": err["plain"], "err_closure": err["closure"],
+        "gpu_s_dns": t_dns.t, "gpu_s_closure": times["closure"], "gpu_s_plain": times["plain"],
+        "speedup_vs_dns": t_dns.t / max(times["closure"], 1e-12),
+        "steps": steps,
+        "Z_dns": zd[: kmax + 1].tolist(), "Z_plain": zp[: kmax + 1].tolist(),
+        "Z_closure": zc[: kmax + 1].tolist(),
+    }
+
+
+def enstrophy_spectrum_of(e, run: int, nu: float = 0.0):
+    """Z(k) = k^2 E(k) of one ensemble member (enstrophy_spectrum's)."""
+    ek = e.spectrum(run, nu)[0]
+    k = np.arange(ek.size, dtype=np.float64)
+    return k * k * ek
+
+
+def _evaluate_per_member(a, net, log=log, extra=None):
     from .closure import ClosedRun, FilteredSampler, Timer, advance_to, enstrophy_spectrum
     from .closure import spectrum_error
     extra = dict(extra or {})
@@ -214,6 +320,8 @@ def main(argv=None):
                     help="1: keep only the enstrophy-sink part of tau (omega tau > 0)")
     ap.add_argument("--guard", type=int, default=1,
                     help="1: zero the closure forcing on backscattering shells")
+    ap.add_argument("--per-member", action="store_true",
+                    help="evaluate the coarse runs member by member instead of as ensembles")
     ap.add_argument("--closure-every", type=int, default=8,
                     help="closure evaluations every this many coarse steps (8: chosen on "
                          "validation seeds 4000.. among 4, 8, 12)")
@@ -249,6 +357,7 @@ def main(argv=None):
         net.set_params(params)
         net.set_scales(*scales)
     net.set_clip(a.clip)
+    a.batched = 0 if a.per_member else 1
     metrics["eval"] = evaluate(a, net)
     e = metrics["eval"]
     if "err_closure" in e:

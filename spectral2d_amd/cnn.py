@@ -30,6 +30,7 @@ class ClosureNet:
         self.H, self.W, self.batch_max = H, W, batch_max
         self.n = param_count()
         self.scales = (1.0, 1.0, 1.0)
+        self.clip = False
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -62,6 +63,7 @@ class ClosureNet:
     def set_clip(self, clip: bool):
         """Keep only the enstrophy-sink part of tau (omega * tau > 0)."""
         self._c(L.lib().s2d_cnn_set_clip(self._h, int(bool(clip))))
+        self.clip = bool(clip)
 
     def set_launch_width(self, blocks: int):
         """Every kernel at the fewest blocks >= `blocks` (grid-stride ones at

@@ -86,6 +86,14 @@ class Ensemble:
                                                    float(dt), float(cfl), int(every),
                                                    int(bool(guard))))
 
+    def get_time(self):
+        """(time[nruns], steps[nruns]) of every run."""
+        t = np.zeros(self.nruns, np.float64)
+        n = np.zeros(self.nruns, np.int64)
+        self._c(L.lib().s2d_ensemble_get_time(self._h, self._d(t),
+                                              n.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return t, n
+
     def clear_forcing(self):
         self._c(L.lib().s2d_ensemble_clear_forcing(self._h))
 
