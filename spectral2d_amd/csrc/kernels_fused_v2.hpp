@@ -866,8 +866,7 @@ __device__ __forceinline__ void passC_finish(const cpx<T> (&v)[WaveCfg<T, NP, V>
             const cpx<T> wsv = op.ws[i], w0v = op.w0[i];
             cpx<T> dW = mk<T>((-du.re - dv.re) - sn * wsv.re, (-du.im - dv.im) - sn * wsv.im);
             if (forced) dW = mk<T>(dW.re - op.fv[i].re, dW.im - op.fv[i].im); // closure / forcing
-            // a run at its end time (dt = 0) keeps its state
-            cpx<T> out = dtd == 0.0 ? w0v : rk3_update<T>(a.stage, dt, w0v, wsv, dW);
+            cpx<T> out = rk3_update<T>(a.stage, dt, w0v, wsv, dW);
             out = sel(keep, out, mk<T>((T)0, (T)0));
             if (active && op.row[i]) Wout[col + op.c[i]] = out;
         }
@@ -923,7 +922,10 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     for (int s = 0; s < E; ++s)
         if ((s & 1) != HALF) mine[LdsPad<T>::idx_sum(it, TG * s)] = v[s];
     __syncthreads();
-    passC_finish<T, NP, HALF, V>(v, other, t, kx, active, op, Fc != nullptr, Wout, col, a, dtd, nu);
+    // a run at its end time (dt = 0) stores nothing: its step-start state
+    // stays in Wn (and Ws, which only this run's later stages read, is unused)
+    passC_finish<T, NP, HALF, V>(v, other, t, kx, active && dtd != 0.0, op, Fc != nullptr, Wout,
+                                 col, a, dtd, nu);
     if (HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0) {
         sc[run].dt = dtd;
         sc[run].time = sc[run].time + dtd;
@@ -971,7 +973,7 @@ __device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T
         const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
         if (Fc) dW = mk<T>(dW.re - Fc[col].re, dW.im - Fc[col].im);
         const double dtd = stage_dt(a, sc + run);
-        Wout[col] = dtd == 0.0 ? w0v : rk3_update<T>(a.stage, (T)dtd, w0v, wsv, dW);
+        if (dtd != 0.0) Wout[col] = rk3_update<T>(a.stage, (T)dtd, w0v, wsv, dW);
     }
     for (int c = 1 + threadIdx.x; c <= d.N; c += NT) Wout[col + c] = mk<T>((T)0, (T)0);
 }
