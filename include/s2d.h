@@ -366,8 +366,9 @@ int s2d_ensemble_steps(s2d_ensemble *ens, int nsteps, double dt, double cfl, int
  * B = runs (the network's batch size must allow it); guard != 0: the shell
  * guard (s2d_set_closure_guard). Each run's forcing, and so its trajectory,
  * is bitwise the one s2d_closure_steps computes for that run alone. grid:
- * an FP32 S2D_NO_DEALIASING context of the runs' size; net: a network of
- * that grid. The forcing stays on for later s2d_ensemble_steps until
+ * an FP32 S2D_NO_DEALIASING context of the runs' size (its transform plans
+ * are used, its fields are left alone); net: a network of that grid. At
+ * most 32767 runs. The forcing stays on for later s2d_ensemble_steps until
  * s2d_ensemble_clear_forcing. No reference counterpart. */
 int s2d_ensemble_closure_steps(s2d_ensemble *ens, s2d_ctx *grid, s2d_cnn *net, int nsteps,
                                double dt, double cfl, int every, int guard);

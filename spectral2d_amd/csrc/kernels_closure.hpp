@@ -15,11 +15,15 @@ namespace s2d {
 // the other resolves as two modes: a destination Nyquist row / column
 // gathers both source modes (+k and -k fold onto one sample), and a source
 // Nyquist coefficient is split in halves over the destination's +k and -k.
+// blockIdx.y selects a pair of planes ss / sd elements further on.
 template <typename Ts, typename Td>
 __global__ void k_restrict(const cpx<Ts> *__restrict__ src, int s_cI, int s_ny,
-                           cpx<Td> *__restrict__ dst, int d_cI, int d_ny, long long kcut2)
+                           cpx<Td> *__restrict__ dst, int d_cI, int d_ny, long long kcut2,
+                           long long ss, long long sd)
 {
 #pragma clang fp contract(off)
+    src += (size_t)blockIdx.y * ss;
+    dst += (size_t)blockIdx.y * sd;
     S2D_GRID_LOOP(idx, (size_t)d_cI * d_ny)
     {
         const int i = (int)(idx % d_cI), jd = (int)(idx / d_cI);
@@ -87,13 +91,16 @@ __global__ void k_axpby(const cpx<T> *__restrict__ x, cpx<T> *__restrict__ y, do
 // changes the enstrophy of shell b (kx^2 + ky^2 in [b^2 - b + 1, b^2 + b])
 // at the rate -2 sum w_kx Re(conj(W) F) (w = 1 at kx = 0, 2 elsewhere);
 // shells where that rate is positive (backscatter) get F = 0. One block per
-// shell, fixed reduction order.
+// shell (blockIdx.x) and run (blockIdx.y, s elements apart), fixed reduction
+// order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_shell_guard(cpx<T> *__restrict__ Fc,
                                                      const cpx<T> *__restrict__ Wc, int N,
-                                                     int K, int KP)
+                                                     int K, int KP, long long s)
 {
     __shared__ double red[256];
+    Fc += (size_t)blockIdx.y * s;
+    Wc += (size_t)blockIdx.y * s;
     const long long b = blockIdx.x;
     const long long lo = b == 0 ? 0 : b * b - b + 1, hi = b == 0 ? 0 : b * b + b;
     const int h = N / 2;

@@ -330,9 +330,11 @@ __global__ void k_check_finite(const cpx<T> *__restrict__ a, size_t n, long long
 // c2r/r2c round trip of omega does).
 template <typename T>
 __global__ void k_pack(const cpx<T> *__restrict__ Wref, cpx<T> *__restrict__ Wc, FusedDims d,
-                       const int *__restrict__ trunc_i1)
+                       const int *__restrict__ trunc_i1, long long sref, long long sc)
 {
 #pragma clang fp contract(off)
+    Wref += (size_t)blockIdx.y * sref; // blockIdx.y: one of a batch of runs
+    Wc += (size_t)blockIdx.y * sc;
     S2D_GRID_LOOP(idx, (size_t)d.K * d.KP)
     {
         const int kx = (int)(idx / d.KP), c = (int)(idx % d.KP);
@@ -356,9 +358,12 @@ __global__ void k_pack(const cpx<T> *__restrict__ Wref, cpx<T> *__restrict__ Wc,
 // compact state -> reference-layout W and PSI slots (ComputePsi + Truncate).
 template <typename T>
 __global__ void k_unpack(const cpx<T> *__restrict__ Wc, cpx<T> *__restrict__ Wref,
-                         cpx<T> *__restrict__ Pref, FusedDims d)
+                         cpx<T> *__restrict__ Pref, FusedDims d, long long sc, long long sref)
 {
 #pragma clang fp contract(off)
+    Wc += (size_t)blockIdx.y * sc; // blockIdx.y: one of a batch of runs
+    Wref += (size_t)blockIdx.y * sref;
+    Pref += (size_t)blockIdx.y * sref;
     S2D_GRID_LOOP(idx, (size_t)d.cI * d.Np)
     {
         const int i = (int)(idx % d.cI), j = (int)(idx / d.cI);

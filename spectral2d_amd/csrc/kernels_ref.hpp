@@ -29,13 +29,15 @@ __device__ __forceinline__ cpx<T> div_real(cpx<T> a, double d)
 // Column c2c (along y) of the reference layout: B columns per workgroup,
 // lane = b + B*t so a load of element s reads B adjacent columns of one row.
 // normalize != 0 divides by norm afterwards (TransformR2C, :761-762).
+// blockIdx.y selects a plane ps elements further on (a batch of fields).
 template <typename T, int E, int D>
 __global__ void k_col_fft(cpx<T> *__restrict__ data, int cI, FftPlan p,
                           const cpx<T> *__restrict__ tw, int B, int normalize,
-                          double norm)
+                          double norm, long long ps)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem);
+    data += (size_t)blockIdx.y * ps;
     const int b = threadIdx.x % B, t = threadIdx.x / B;
     const int i = blockIdx.x * B + b;
     const bool active = i < cI;
@@ -57,13 +59,15 @@ __global__ void k_col_fft(cpx<T> *__restrict__ data, int cI, FftPlan p,
 
 // Row r2c of length n = 2h through a length-h complex transform, in place
 // on each row (the real row and its complex row share memory). B rows per
-// workgroup, lane = t + T*b. twx[k] = exp(-2 pi i k / n).
+// workgroup, lane = t + T*b. twx[k] = exp(-2 pi i k / n). blockIdx.y
+// selects a plane ps elements further on.
 template <typename T, int E>
 __global__ void k_row_r2c(cpx<T> *__restrict__ data, int cI, int nrows, FftPlan p,
                           const cpx<T> *__restrict__ tw, const cpx<T> *__restrict__ twx,
-                          int B)
+                          int B, long long ps)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    data += (size_t)blockIdx.y * ps;
     const int h = p.n;
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem) + (size_t)(threadIdx.x / p.T) * h;
     const int t = threadIdx.x % p.T, b = threadIdx.x / p.T;
@@ -102,9 +106,10 @@ __global__ void k_row_r2c(cpx<T> *__restrict__ data, int cI, int nrows, FftPlan 
 template <typename T, int E>
 __global__ void k_row_c2r(cpx<T> *__restrict__ data, int cI, int nrows, FftPlan p,
                           const cpx<T> *__restrict__ tw, const cpx<T> *__restrict__ twx,
-                          int B)
+                          int B, long long ps)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    data += (size_t)blockIdx.y * ps;
     const int h = p.n;
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem) + (size_t)(threadIdx.x / p.T) * h;
     const int t = threadIdx.x % p.T, b = threadIdx.x / p.T;

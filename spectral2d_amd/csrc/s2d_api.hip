@@ -246,8 +246,9 @@ void launch(s2d_ctx *c, K kern, dim3 grid, dim3 block, size_t shmem, Args... arg
 
 // ----------------------------------------------- reference-layout FFTs
 
+// np planes ps elements apart (one by default)
 template <typename T, int E, int D>
-int launch_col_E(s2d_ctx *c, cpx<T> *data, int normalize)
+int launch_col_E(s2d_ctx *c, cpx<T> *data, int normalize, int np, long long ps)
 {
     const FftPlan &p = c->col.plan;
     int B = 1024 / p.T;
@@ -260,28 +261,28 @@ int launch_col_E(s2d_ctx *c, cpx<T> *data, int normalize)
     allow_lds(kern, lds);
     const int blocks = (c->g.cISize + B - 1) / B;
     const double norm = (double)c->g.nxG * (double)c->g.nyG;
-    launch(c, kern, dim3(blocks), dim3(B * p.T), lds, data, c->g.cISize, p,
-           (const cpx<T> *)c->col.tw, B, normalize, norm);
+    launch(c, kern, dim3(blocks, np), dim3(B * p.T), lds, data, c->g.cISize, p,
+           (const cpx<T> *)c->col.tw, B, normalize, norm, ps);
     S2D_LAUNCHED(c, "k_col_fft");
     return S2D_OK;
 }
 
 template <typename T, int D>
-int launch_col(s2d_ctx *c, cpx<T> *data, int normalize)
+int launch_col(s2d_ctx *c, cpx<T> *data, int normalize, int np = 1, long long ps = 0)
 {
     switch (c->col.E) {
-    case 12: return launch_col_E<T, 12, D>(c, data, normalize);
-    case 8: return launch_col_E<T, 8, D>(c, data, normalize);
-    case 6: return launch_col_E<T, 6, D>(c, data, normalize);
-    case 4: return launch_col_E<T, 4, D>(c, data, normalize);
-    case 3: return launch_col_E<T, 3, D>(c, data, normalize);
-    case 2: return launch_col_E<T, 2, D>(c, data, normalize);
+    case 12: return launch_col_E<T, 12, D>(c, data, normalize, np, ps);
+    case 8: return launch_col_E<T, 8, D>(c, data, normalize, np, ps);
+    case 6: return launch_col_E<T, 6, D>(c, data, normalize, np, ps);
+    case 4: return launch_col_E<T, 4, D>(c, data, normalize, np, ps);
+    case 3: return launch_col_E<T, 3, D>(c, data, normalize, np, ps);
+    case 2: return launch_col_E<T, 2, D>(c, data, normalize, np, ps);
     }
     return fail(c, S2D_ERR_UNSUPPORTED, "no column kernel for E=%d", c->col.E);
 }
 
 template <typename T, int E, bool R2C>
-int launch_row_E(s2d_ctx *c, cpx<T> *data)
+int launch_row_E(s2d_ctx *c, cpx<T> *data, int np, long long ps)
 {
     const FftPlan &p = c->rowh.plan;
     int B = 256 / p.T;
@@ -294,28 +295,28 @@ int launch_row_E(s2d_ctx *c, cpx<T> *data)
     if constexpr (R2C) {
         auto kern = k_row_r2c<T, E>;
         allow_lds(kern, lds);
-        launch(c, kern, dim3(blocks), dim3(B * p.T), lds, data, c->g.cISize, c->g.nyG, p,
-               (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B);
+        launch(c, kern, dim3(blocks, np), dim3(B * p.T), lds, data, c->g.cISize, c->g.nyG, p,
+               (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B, ps);
     } else {
         auto kern = k_row_c2r<T, E>;
         allow_lds(kern, lds);
-        launch(c, kern, dim3(blocks), dim3(B * p.T), lds, data, c->g.cISize, c->g.nyG, p,
-               (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B);
+        launch(c, kern, dim3(blocks, np), dim3(B * p.T), lds, data, c->g.cISize, c->g.nyG, p,
+               (const cpx<T> *)c->rowh.tw, (const cpx<T> *)c->tw_x, B, ps);
     }
     S2D_LAUNCHED(c, R2C ? "k_row_r2c" : "k_row_c2r");
     return S2D_OK;
 }
 
 template <typename T, bool R2C>
-int launch_row(s2d_ctx *c, cpx<T> *data)
+int launch_row(s2d_ctx *c, cpx<T> *data, int np = 1, long long ps = 0)
 {
     switch (c->rowh.E) {
-    case 12: return launch_row_E<T, 12, R2C>(c, data);
-    case 8: return launch_row_E<T, 8, R2C>(c, data);
-    case 6: return launch_row_E<T, 6, R2C>(c, data);
-    case 4: return launch_row_E<T, 4, R2C>(c, data);
-    case 3: return launch_row_E<T, 3, R2C>(c, data);
-    case 2: return launch_row_E<T, 2, R2C>(c, data);
+    case 12: return launch_row_E<T, 12, R2C>(c, data, np, ps);
+    case 8: return launch_row_E<T, 8, R2C>(c, data, np, ps);
+    case 6: return launch_row_E<T, 6, R2C>(c, data, np, ps);
+    case 4: return launch_row_E<T, 4, R2C>(c, data, np, ps);
+    case 3: return launch_row_E<T, 3, R2C>(c, data, np, ps);
+    case 2: return launch_row_E<T, 2, R2C>(c, data, np, ps);
     }
     return fail(c, S2D_ERR_UNSUPPORTED, "no row kernel for E=%d", c->rowh.E);
 }
@@ -515,7 +516,8 @@ int op_pack(s2d_ctx *c, int run = 0)
     FusedDims d = fdims(c);
     size_t n = (size_t)d.K * d.KP;
     hipLaunchKernelGGL(k_pack<T>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
-                       F<T>(c, S2D_FIELD_W), (cpx<T> *)c->Wn + run * c->sW, d, c->d_trunc);
+                       F<T>(c, S2D_FIELD_W), (cpx<T> *)c->Wn + run * c->sW, d, c->d_trunc, 0LL,
+                       0LL);
     S2D_LAUNCHED(c, "k_pack");
     return S2D_OK;
 }
@@ -527,7 +529,7 @@ int op_unpack(s2d_ctx *c, int run = 0)
     size_t n = nc_of(c);
     hipLaunchKernelGGL(k_unpack<T>, dim3(grid_blocks(n, 256)), dim3(256), 0, c->stream,
                        (const cpx<T> *)c->Wn + run * c->sW, F<T>(c, S2D_FIELD_W),
-                       F<T>(c, S2D_FIELD_PSI), d);
+                       F<T>(c, S2D_FIELD_PSI), d, 0LL, 0LL);
     S2D_LAUNCHED(c, "k_unpack");
     return S2D_OK;
 }
@@ -1031,7 +1033,7 @@ int closure_restrict(s2d_ctx *c, s2d_ctx *src, int sf, s2d_ctx *dst, int df)
     const size_t n = nc_of(dst);
     hipLaunchKernelGGL((k_restrict<float, float>), dim3(grid_blocks(n, 256)), dim3(256), 0,
                        c->stream, F<float>(src, sf), src->g.cISize, src->g.nyG,
-                       F<float>(dst, df), dst->g.cISize, dst->g.nyG, kc * kc);
+                       F<float>(dst, df), dst->g.cISize, dst->g.nyG, kc * kc, 0LL, 0LL);
     return hipGetLastError() == hipSuccess ? S2D_OK
                                            : fail(c, S2D_ERR_HIP, "launch of k_restrict failed");
 }
@@ -1068,7 +1070,7 @@ int closure_output(s2d_ctx *c, s2d_ctx *g, int run)
         const int nshell = (int)std::floor(std::sqrt(2.0) * (c->N / 2)) + 2;
         hipLaunchKernelGGL(k_shell_guard<float>, dim3(nshell), dim3(256), 0, c->stream,
                            (cpx<float> *)c->forcing + run * c->sW,
-                           (const cpx<float> *)c->Wn + run * c->sW, c->N, c->K, c->KP);
+                           (const cpx<float> *)c->Wn + run * c->sW, c->N, c->K, c->KP, 0LL);
         if (hipGetLastError() != hipSuccess) r = fail(c, S2D_ERR_HIP, "launch of k_shell_guard failed");
     }
     return r;
@@ -1802,16 +1804,20 @@ int s2d_restrict(s2d_ctx *src, int sf, s2d_ctx *dst, int df, double kcut)
     const dim3 grid(grid_blocks(n, 256)), blk(256);
     if (src->precision == S2D_FP64 && dst->precision == S2D_FP64)
         hipLaunchKernelGGL((k_restrict<double, double>), grid, blk, 0, dst->stream,
-                           F<double>(src, sf), sc, sn, F<double>(dst, df), dc, dn, kc * kc);
+                           F<double>(src, sf), sc, sn, F<double>(dst, df), dc, dn, kc * kc,
+                           0LL, 0LL);
     else if (src->precision == S2D_FP64)
         hipLaunchKernelGGL((k_restrict<double, float>), grid, blk, 0, dst->stream,
-                           F<double>(src, sf), sc, sn, F<float>(dst, df), dc, dn, kc * kc);
+                           F<double>(src, sf), sc, sn, F<float>(dst, df), dc, dn, kc * kc,
+                           0LL, 0LL);
     else if (dst->precision == S2D_FP64)
         hipLaunchKernelGGL((k_restrict<float, double>), grid, blk, 0, dst->stream,
-                           F<float>(src, sf), sc, sn, F<double>(dst, df), dc, dn, kc * kc);
+                           F<float>(src, sf), sc, sn, F<double>(dst, df), dc, dn, kc * kc,
+                           0LL, 0LL);
     else
         hipLaunchKernelGGL((k_restrict<float, float>), grid, blk, 0, dst->stream,
-                           F<float>(src, sf), sc, sn, F<float>(dst, df), dc, dn, kc * kc);
+                           F<float>(src, sf), sc, sn, F<float>(dst, df), dc, dn, kc * kc,
+                           0LL, 0LL);
     S2D_LAUNCHED(dst, "k_restrict");
     wrote(dst, df);
     return sync(dst);
