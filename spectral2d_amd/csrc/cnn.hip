@@ -171,6 +171,12 @@ __global__ __launch_bounds__(256) void k_conv_in(const float *__restrict__ x0,
         }
         av[mt] = as_bf8(uint4{w[0], w[1], w[2], w[3]});
     }
+    // the bias before any store (vmcnt counts stores too: a load issued
+    // after the stores would wait for all of them)
+    float4 bb[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+        bb[mt] = *reinterpret_cast<const float4 *>(P + OFF_B0 + 16 * mt + 4 * g4);
     __syncthreads();
     // B: lane (pixel 32 w + 16 nt + l16, k = 8 g4 + e): the pair of tap (8 g4 + e) / 2
     float4_t acc[4][2];
@@ -195,9 +201,8 @@ __global__ __launch_bounds__(256) void k_conv_in(const float *__restrict__ x0,
         const size_t pix = ((size_t)row * W + xb + 32 * wv + 16 * nt + l16) * CH + 4 * g4;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
-            const float4 bb = *reinterpret_cast<const float4 *>(P + OFF_B0 + 16 * mt + 4 * g4);
-            const float v[4] = {acc[mt][nt][0] + bb.x, acc[mt][nt][1] + bb.y,
-                                acc[mt][nt][2] + bb.z, acc[mt][nt][3] + bb.w};
+            const float v[4] = {acc[mt][nt][0] + bb[mt].x, acc[mt][nt][1] + bb[mt].y,
+                                acc[mt][nt][2] + bb[mt].z, acc[mt][nt][3] + bb[mt].w};
             const size_t o = pix + 16 * mt;
             if (z) *reinterpret_cast<uint2 *>(z + o) = uint2{pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3])};
             *reinterpret_cast<uint2 *>(a + o) =
@@ -231,6 +236,7 @@ __global__ __launch_bounds__(CV_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 {
     __shared__ uint4 sw[9 * CH * 8];             // 72 KiB
     __shared__ uint4 sr[CV_RING * CV_ROWCH];     // 82.5 KiB
+    __shared__ __align__(16) float sb[CH];       // the bias (zeros without one)
     const int segs = W / CV_SEG, nyb = H / R;
     int b = blockIdx.x;
     const int seg = b % segs;
@@ -272,16 +278,17 @@ __global__ __launch_bounds__(CV_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                 sr[sw_chunk(i * CV_PX + (r >> 3), r & 7)] = vr[k];
             }
         }
+        if (tid < CH) sb[tid] = bias ? bias[tid] : 0.f;
     }
     __syncthreads();
     // Waves 0-3 (group 0) and 4-7 (group 1) share the SIMDs one to one and run
     // in opposite phases, so that one wave's MFMAs overlap the other's
     // epilogue (VALU) on every SIMD. Step s, output rows y0 + 8 s + 4 grp + wr:
-    //   phase 1: group 0 MFMAs; group 1 the epilogue of its rows of step s - 1
-    //            and the loads of ring rows 8 s + 10 .. 8 s + 13
-    //   phase 2: group 1 stores those rows, then its MFMAs; group 0 its
-    //            epilogue and the loads of ring rows 8 s + 14 .. 8 s + 17,
-    //            which it stores at the start of step s + 1.
+    //   phase 1: group 0 MFMAs; group 1 the loads of ring rows 8 s + 10 ..
+    //            8 s + 13 and the epilogue of its rows of step s - 1
+    //   phase 2: group 1 stores those rows, then its MFMAs; group 0 the
+    //            loads of ring rows 8 s + 14 .. 8 s + 17 (stored at the start
+    //            of step s + 1) and its epilogue.
     // Ring rows 8 s + 10 .. 13 take the slots of rows 8 s .. 8 s + 3 (last read
     // by group 0 in phase 1), rows 8 s + 14 .. 17 those of 8 s + 4 .. 8 s + 7
     // (last read by group 1 in phase 2); a barrier ends each phase.
@@ -343,33 +350,44 @@ __global__ __launch_bounds__(CV_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);                          // MFMAs
         });
     };
-    // lane: channels co = 16 mt + 4 g4 + i of pixel x0 + 16 nt + l16 of image row y
-    auto epilogue = [&](int y) {
+    // lane: channels co = 16 mt + 4 g4 + i of pixel x0 + 16 nt + l16 of image row y.
+    // No global load follows a store here: vmcnt counts both, so a load
+    // issued after the stores would wait for all of them (measured: 24 %
+    // of the forward time, with a bias load per tile). The bias comes from
+    // LDS; the backward pass loads its 16 z values before the first store.
+    auto epilogue = [&](int y) __attribute__((always_inline)) {
+        const size_t pix0 = (((size_t)n * H + y) * W + x0 + l16) * CH + 4 * g4;
+        if (mode == 0) {
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const size_t pix = (((size_t)n * H + y) * W + x0 + 16 * nt + l16) * CH + 4 * g4;
+            for (int nt = 0; nt < 4; ++nt) {
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-                const size_t o = pix + 16 * mt;
-                float4 bb = float4{0.f, 0.f, 0.f, 0.f};
-                if (bias) bb = *reinterpret_cast<const float4 *>(bias + 16 * mt + 4 * g4);
-                const float v[4] = {acc[mt][nt][0] + bb.x, acc[mt][nt][1] + bb.y,
-                                    acc[mt][nt][2] + bb.z, acc[mt][nt][3] + bb.w};
-                if (mode == 0) {
+                for (int mt = 0; mt < 4; ++mt) {
+                    const size_t o = pix0 + (size_t)16 * nt * CH + 16 * mt;
+                    const float4 bb = *reinterpret_cast<const float4 *>(sb + 16 * mt + 4 * g4);
+                    const float v0 = acc[mt][nt][0] + bb.x, v1 = acc[mt][nt][1] + bb.y,
+                                v2 = acc[mt][nt][2] + bb.z, v3 = acc[mt][nt][3] + bb.w;
                     if (zo)
-                        *reinterpret_cast<uint2 *>(zo + o) =
-                            uint2{pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3])};
+                        *reinterpret_cast<uint2 *>(zo + o) = uint2{pk_bf16(v0, v1), pk_bf16(v2, v3)};
                     *reinterpret_cast<uint2 *>(ao + o) =
-                        uint2{pk_bf16(gelu(v[0]), gelu(v[1])), pk_bf16(gelu(v[2]), gelu(v[3]))};
-                } else {
-                    const uint2 zz = *reinterpret_cast<const uint2 *>(zp + o);
-                    const u16 *zh = reinterpret_cast<const u16 *>(&zz);
-                    float g[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) g[i] = v[i] * gelu_grad(bf2f(zh[i]));
-                    *reinterpret_cast<uint2 *>(zo + o) =
-                        uint2{pk_bf16(g[0], g[1]), pk_bf16(g[2], g[3])};
+                        uint2{pk_bf16(gelu(v0), gelu(v1)), pk_bf16(gelu(v2), gelu(v3))};
                 }
+            }
+        } else {
+            uint2 zz[16]; // tile q: nt = q / 4, mt = q % 4
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                zz[q] = *reinterpret_cast<const uint2 *>(zp + pix0 + (size_t)16 * (q >> 2) * CH +
+                                                         16 * (q & 3));
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int nt = q >> 2, mt = q & 3;
+                const float4_t a = acc[mt][nt];
+                const float g0 = a[0] * gelu_grad(bf2f((u16)(zz[q].x & 0xffffu)));
+                const float g1 = a[1] * gelu_grad(bf2f((u16)(zz[q].x >> 16)));
+                const float g2 = a[2] * gelu_grad(bf2f((u16)(zz[q].y & 0xffffu)));
+                const float g3 = a[3] * gelu_grad(bf2f((u16)(zz[q].y >> 16)));
+                *reinterpret_cast<uint2 *>(zo + pix0 + (size_t)16 * nt * CH + 16 * mt) =
+                    uint2{pk_bf16(g0, g1), pk_bf16(g2, g3)};
             }
         }
     };
@@ -379,9 +397,9 @@ __global__ __launch_bounds__(CV_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         if (grp == 0) {
             if (s > 0) store_rows(8 * s + 6); // rows 8 (s - 1) + 14 .. 17
             mfma_row(8 * s + wr);
-        } else {
-            if (s > 0) epilogue(y0 + 8 * (s - 1) + 4 + wr);
+        } else { // (the loads ahead of the stores: their wait leaves the stores in flight)
             if (more) load_rows(8 * s + 10);
+            if (s > 0) epilogue(y0 + 8 * (s - 1) + 4 + wr);
         }
         __syncthreads();
         // phase 2
@@ -389,8 +407,8 @@ __global__ __launch_bounds__(CV_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             if (more) store_rows(8 * s + 10);
             mfma_row(8 * s + 4 + wr);
         } else {
-            epilogue(y0 + 8 * s + wr);
             if (more) load_rows(8 * s + 14);
+            epilogue(y0 + 8 * s + wr);
         }
         __syncthreads();
     }
@@ -617,41 +635,58 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y,
 
 // Backward of layer 5: dZ4[q][ci] = (sum_t dy[q - off_t] W5[t][ci]) *
 // gelu'(z4[q][ci]) (BF16). Eight lanes per pixel, eight channels each; a
-// lane keeps its channels' 72 weights in registers over a grid-stride loop.
+// block takes OB_PX pixels, OB_PPT per lane, whose loads are all issued
+// before the first store (a loop with a store between loads waits for the
+// store at every load: vmcnt counts both). The rounded W5 sits in LDS.
+constexpr int OB_PPT = 4, OB_PX = 256 / 8 * OB_PPT;
 __global__ __launch_bounds__(256) void k_out_bwd_data(const float *__restrict__ dy,
                                                       const float *__restrict__ P,
                                                       const u16 *__restrict__ z4,
                                                       u16 *__restrict__ dz, int B, int H, int W)
 {
-    const int c8 = 8 * (threadIdx.x & 7);
-    float w[9][8];
+    __shared__ __align__(16) float ws[9 * CH];
+    for (int i = threadIdx.x; i < 9 * CH; i += 256) ws[i] = bfr(P[OFF_W5 + i]);
+    const int c8 = 8 * (threadIdx.x & 7), npx = B * H * W; // pixels < 2^31 (s2d_cnn_create)
+    int qs[OB_PPT];
+    uint4 zz[OB_PPT];
+    float d[OB_PPT][9];
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) w[t][k] = bfr(P[OFF_W5 + t * CH + c8 + k]);
-    const long long npx = (long long)B * H * W;
-    for (long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 3; q < npx;
-         q += ((long long)gridDim.x * blockDim.x) >> 3) {
-        const int xx = (int)(q % W), yy = (int)((q / W) % H), n = (int)(q / ((long long)W * H));
-        const uint4 zz = *reinterpret_cast<const uint4 *>(z4 + q * CH + c8);
-        float d[9];
+    for (int j = 0; j < OB_PPT; ++j) {
+        qs[j] = blockIdx.x * OB_PX + 32 * j + (int)(threadIdx.x >> 3);
+        const int q = qs[j] < npx ? qs[j] : npx - 1; // (past the end: loaded, not stored)
+        const int xx = q % W, yy = (q / W) % H, n = q / (W * H);
+        zz[j] = *reinterpret_cast<const uint4 *>(z4 + (size_t)q * CH + c8);
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int y2 = wrap(yy - (t / 3 - 1), H), x2 = wrap(xx - (t % 3 - 1), W);
-            d[t] = dy[((size_t)n * H + y2) * W + x2];
+            d[j][t] = dy[((size_t)n * H + y2) * W + x2];
         }
-        const u16 *zh = reinterpret_cast<const u16 *>(&zz);
+    }
+    __syncthreads();
+    float w[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const float4 lo = *reinterpret_cast<const float4 *>(ws + t * CH + c8);
+        const float4 hi = *reinterpret_cast<const float4 *>(ws + t * CH + c8 + 4);
+        w[t][0] = lo.x, w[t][1] = lo.y, w[t][2] = lo.z, w[t][3] = lo.w;
+        w[t][4] = hi.x, w[t][5] = hi.y, w[t][6] = hi.z, w[t][7] = hi.w;
+    }
+#pragma unroll
+    for (int j = 0; j < OB_PPT; ++j) {
+        const uint32_t zw[4] = {zz[j].x, zz[j].y, zz[j].z, zz[j].w};
         float o[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             float s = 0.f;
 #pragma unroll
-            for (int t = 0; t < 9; ++t) s += d[t] * w[t][k];
-            o[k] = s * gelu_grad(bf2f(zh[k]));
+            for (int t = 0; t < 9; ++t) s += d[j][t] * w[t][k];
+            const u16 zh = (u16)((k & 1) ? zw[k >> 1] >> 16 : zw[k >> 1] & 0xffffu);
+            o[k] = s * gelu_grad(bf2f(zh));
         }
-        *reinterpret_cast<uint4 *>(dz + q * CH + c8) =
-            uint4{pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]), pk_bf16(o[4], o[5]),
-                  pk_bf16(o[6], o[7])};
+        if (qs[j] < npx)
+            *reinterpret_cast<uint4 *>(dz + (size_t)qs[j] * CH + c8) =
+                uint4{pk_bf16(o[0], o[1]), pk_bf16(o[2], o[3]), pk_bf16(o[4], o[5]),
+                      pk_bf16(o[6], o[7])};
     }
 }
 
@@ -975,11 +1010,8 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
                        rth, c->part);
     CNN_TRY(launched(c, "k_out_bwd_w"));
     CNN_TRY(reduce(nth, 9 * CH + 1, c->G + OFF_W5));
-    // a grid-stride loop over a few waves of blocks: each lane loads its 72 weights once
-    hipLaunchKernelGGL(k_out_bwd_data,
-                       dim3(std::min(grid_for(npx * 8, 256), (unsigned)width_of(c, 2048))), dim3(256),
-                       0, c->stream,
-                       c->dy, c->P, c->z[NMID], c->dz[0], B, H, W);
+    hipLaunchKernelGGL(k_out_bwd_data, dim3((unsigned)((npx + OB_PX - 1) / OB_PX)), dim3(256), 0,
+                       c->stream, c->dy, c->P, c->z[NMID], c->dz[0], B, H, W);
     CNN_TRY(launched(c, "k_out_bwd_data"));
     int cur = 0;
     const int rdw = dw_rows(B, H, W, width_of(c, 512)), ndw = (int)dw_blocks(B, H, W, width_of(c, 512));
@@ -1040,6 +1072,8 @@ int s2d_cnn_create(int H, int W, int batch_max, s2d_cnn **out)
     if (H < 8 || W < WSEG || W % WSEG || H % 8 || batch_max < 1)
         return cfail(nullptr, S2D_ERR_UNSUPPORTED,
                      "grid %dx%d: W a multiple of %d, H a multiple of 8, batch >= 1", H, W, WSEG);
+    if ((long long)batch_max * H * W >= (1LL << 31)) // pixel indices are 32-bit
+        return cfail(nullptr, S2D_ERR_UNSUPPORTED, "batch x grid of 2^31 pixels or more");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return cfail(nullptr, S2D_ERR_NO_DEVICE, "no GPU visible");
