@@ -282,11 +282,6 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     const int kx0 = (FIELD_FAST ? rem / 3 : rem - f * tiles) * LPB;
     const int kx = kx0 + g;
     const bool active = kx < d.K;
-    if (reset_max && f == 0 && kx0 == 0 && threadIdx.x == 0) {
-        sc[run].vmax_bits = 0ull;
-        sc[run].t_start = sc[run].time;
-        sc[run].bad = 0u;
-    }
     const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
     const T qx = (T)kx;
     // every load of the column first (unconditional, in bounds), then the
@@ -302,6 +297,28 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
     });
     GroupTw<T, E, TG, C::TW_PRE_A> gt;
     gt.load(tw, t);
+    // the stage's scalar reset after the loads (vmcnt counts loads and
+    // stores in order: stores ahead of the loads would delay their waits)
+    if (reset_max && f == 0 && kx0 == 0 && threadIdx.x == 0) {
+        sc[run].vmax_bits = 0ull;
+        sc[run].t_start = sc[run].time;
+        sc[run].bad = 0u;
+    }
+    if (passA_tail(d.K, LPB) && kx0 + LPB == d.K - 1) { // block-uniform
+        // column K - 1 of field f: its ky = 0 mode (compact row 0) taken
+        // through the same spectrum arithmetic as a transformed column;
+        // written here, ahead of the transform (its load then waits for no
+        // store)
+        const int kt = d.K - 1;
+        const cpx<T> w0 = Wsrc[run * bt.sW + (size_t)kt * d.KP];
+        const T qt = (T)kt;
+        const T inv = recip(qt * qt);
+        const cpx<T> ps = mk<T>(w0.re * inv, w0.im * inv);
+        const T a = (f == 0) ? (T)0 : -qt;
+        const cpx<T> val = f == 2 ? w0 : mk<T>(-(a * ps.im), a * ps.re);
+        cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kt;
+        for (int y = threadIdx.x; y < NP; y += BLK) col[(size_t)y * d.KG] = val;
+    }
     cpx<T> v[E];
     if (f == 2) { // block-uniform
         static_for<0, E>([&](auto sI) {
@@ -347,19 +364,6 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
             col[(size_t)y * d.KG] = sl[LdsPad<T>::idx(y)];
         }
     }
-    if (passA_tail(d.K, LPB) && kx0 + LPB == d.K - 1) { // block-uniform
-        // column K - 1 of field f: its ky = 0 mode (compact row 0) taken
-        // through the same spectrum arithmetic as a transformed column
-        const int kt = d.K - 1;
-        const cpx<T> w0 = Wsrc[run * bt.sW + (size_t)kt * d.KP];
-        const T qx = (T)kt;
-        const T inv = recip(qx * qx);
-        const cpx<T> ps = mk<T>(w0.re * inv, w0.im * inv);
-        const T a = (f == 0) ? (T)0 : -qx;
-        const cpx<T> val = f == 2 ? w0 : mk<T>(-(a * ps.im), a * ps.re);
-        cpx<T> *col = (f == 0 ? Gu : f == 1 ? Gv : Gw) + run * bt.sG + kt;
-        for (int y = threadIdx.x; y < NP; y += BLK) col[(size_t)y * d.KG] = val;
-    }
 }
 
 // Pass A with the three fields of a column tile in one block (VAR_FUSE_A):
@@ -386,11 +390,6 @@ __device__ __forceinline__ void passA_fused_body(int blk, int nblk, cpx<T> *lds,
     const int run = job / tiles, kx0 = (job - run * tiles) * LPB;
     const int kx = kx0 + g;
     const bool active = kx < d.K;
-    if (reset_max && kx0 == 0 && threadIdx.x == 0) {
-        sc[run].vmax_bits = 0ull;
-        sc[run].t_start = sc[run].time;
-        sc[run].bad = 0u;
-    }
     const cpx<T> *src = Wsrc + run * bt.sW + (size_t)(active ? kx : 0) * d.KP;
     const T qx = (T)kx;
     cpx<T> x[E];
@@ -404,6 +403,11 @@ __device__ __forceinline__ void passA_fused_body(int blk, int nblk, cpx<T> *lds,
     });
     GroupTw<T, E, TG, C::TW_PRE_A> gt;
     gt.load(tw, t);
+    if (reset_max && kx0 == 0 && threadIdx.x == 0) { // (after the loads, as passA_body)
+        sc[run].vmax_bits = 0ull;
+        sc[run].t_start = sc[run].time;
+        sc[run].bad = 0u;
+    }
     cpx<T> ps[E], v[E];
     static_for<0, E>([&](auto sI) {
         constexpr int s = decltype(sI)::value;
@@ -891,7 +895,8 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
                                            const cpx<T> *__restrict__ Hsrc, const cpx<T> *Wn,
                                            const cpx<T> *Wst, cpx<T> *Wout, const FusedDims &d,
                                            const cpx<T> *__restrict__ tw, DevScalars *sc,
-                                           const StageArgs &a, const Batch &bt)
+                                           const StageArgs &a, const Batch &bt,
+                                           double &dt_out, double &nu_out)
 {
     using C = WaveCfg<T, NP, V>;
     constexpr int E = C::E, TG = C::TG;
@@ -914,6 +919,22 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     gt.load(tw, t);
     const double dtd = stage_dt(a, sc + run);
     const double nu = bt.nu ? bt.nu[run] : a.nu;
+    dt_out = dtd;
+    nu_out = nu;
+    // the run's step-end scalars: one counter, vmcnt, tracks loads and
+    // stores in order, so loads issued after the column's stores wait for
+    // all of them. The large transforms (the sizes that run in one round,
+    // with registers to spare) load them before the stores; the others
+    // (five waves per SIMD in FP32) after.
+    constexpr bool SC_EARLY = TG >= 128;
+    const bool upd = HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0;
+    double t_old = 0.0, t_end = 0.0;
+    long long n_old = 0;
+    if (SC_EARLY && upd) {
+        t_old = sc[run].time;
+        t_end = sc[run].t_end;
+        n_old = sc[run].nadv;
+    }
     group_fft<T, E, TG, -1>(v, mine, t, gt);
     if constexpr (!C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
     // half 0 keeps the even s and hands over the odd s, half 1 the reverse
@@ -926,10 +947,15 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     // stays in Wn (and Ws, which only this run's later stages read, is unused)
     passC_finish<T, NP, HALF, V>(v, other, t, kx, active && dtd != 0.0, op, Fc != nullptr, Wout,
                                  col, a, dtd, nu);
-    if (HALF == 0 && a.stage == 3 && active && kx == 0 && t == 0) {
+    if (upd) {
+        if constexpr (!SC_EARLY) {
+            t_old = sc[run].time;
+            t_end = sc[run].t_end;
+            n_old = sc[run].nadv;
+        }
         sc[run].dt = dtd;
-        sc[run].time = sc[run].time + dtd;
-        if (!(dtd == 0.0 && sc[run].t_end > 0.0)) sc[run].nadv = sc[run].nadv + 1;
+        sc[run].time = t_old + dtd;
+        if (!(dtd == 0.0 && t_end > 0.0)) sc[run].nadv = n_old + 1;
     }
 }
 
@@ -943,22 +969,40 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
 template <typename T, int NP, int V = 0>
 __device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T> *Huw,
                                            const cpx<T> *Wn, const cpx<T> *Wst, cpx<T> *Wout,
-                                           const FusedDims &d, DevScalars *sc,
-                                           const StageArgs &a, const Batch &bt)
+                                           const FusedDims &d, const StageArgs &a,
+                                           const Batch &bt, double dtd, double nu_run)
 {
 #pragma clang fp contract(off)
     constexpr int NT = 2 * WaveCfg<T, NP, V>::BLK, NW = NT / 64;
+    static_assert(WaveCfg<T, NP, V>::GPB == 1, "tail: one column per half (its run's column 0)");
     const int kx = d.K - 1;
     const cpx<T> *h = Huw + run * bt.sH + (size_t)kx * NP;
+    const size_t col = run * bt.sW + (size_t)kx * d.KP;
+    const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
+    // every load of the tail issued together, after the column's stores,
+    // which their one wait includes (vmcnt counts both); the run's dt and
+    // viscosity come from the column's own pass (thread 0: half 0, the
+    // run's column 0)
+    cpx<T> wsv = mk<T>((T)0, (T)0), w0v = wsv, fcv = wsv;
+    if (threadIdx.x == 0) {
+        wsv = Wst[col];
+        w0v = Wn[col];
+        if (Fc) fcv = Fc[col];
+    }
+    const T nu = (T)nu_run;
+    static_assert(NP % NT == 0, "tail: whole strides");
+    cpx<T> hv[NP / NT];
+#pragma unroll
+    for (int k = 0; k < NP / NT; ++k) hv[k] = h[threadIdx.x + NT * k];
     cpx<T> p = mk<T>((T)0, (T)0);
-    for (int y = threadIdx.x; y < NP; y += NT) p = add(p, h[y]);
+#pragma unroll
+    for (int k = 0; k < NP / NT; ++k) p = add(p, hv[k]); // y = threadIdx.x + NT k, in order
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
         p = mk<T>(p.re + __shfl_xor(p.re, off), p.im + __shfl_xor(p.im, off));
     __syncthreads(); // the slices' last reads (passC_finish) are done
     if ((threadIdx.x & 63) == 0) lds_all[threadIdx.x / 64] = p;
     __syncthreads();
-    const size_t col = run * bt.sW + (size_t)kx * d.KP;
     if (threadIdx.x == 0) {
         cpx<T> P = lds_all[0];
 #pragma unroll
@@ -967,12 +1011,9 @@ __device__ __forceinline__ void passC_tail(int run, cpx<T> *lds_all, const cpx<T
         P = mk<T>(P.re * inv_norm, P.im * inv_norm);
         const T qx = (T)kx;
         const cpx<T> du = mk<T>(-(qx * P.im), qx * P.re); // DuDx of (uw)^
-        const T sn = (T)(bt.nu ? bt.nu[run] : a.nu) * (qx * qx);
-        const cpx<T> wsv = Wst[col], w0v = Wn[col];
+        const T sn = nu * (qx * qx);
         cpx<T> dW = mk<T>((-du.re) - sn * wsv.re, (-du.im) - sn * wsv.im);
-        const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
-        if (Fc) dW = mk<T>(dW.re - Fc[col].re, dW.im - Fc[col].im);
-        const double dtd = stage_dt(a, sc + run);
+        if (Fc) dW = mk<T>(dW.re - fcv.re, dW.im - fcv.im);
         if (dtd != 0.0) Wout[col] = rk3_update<T>(a.stage, (T)dtd, w0v, wsv, dW);
     }
     for (int c = 1 + threadIdx.x; c <= d.N; c += NT) Wout[col + c] = mk<T>((T)0, (T)0);
@@ -992,16 +1033,19 @@ __device__ __forceinline__ void passC_body(int blk, cpx<T> *lds_all,
     const int th = tid < 0 ? (int)threadIdx.x : tid;
     const int half = th / C::BLK, lane = th % C::BLK;
     // wave-uniform; both halves pass the same barriers
+    double dtd = 0.0, nu = 0.0;
     if (half == 0)
-        passC_half<T, NP, 0, TAIL, V>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt);
+        passC_half<T, NP, 0, TAIL, V>(blk, lds_all, lane, Huw, Wn, Wst, Wout, d, tw, sc, a, bt,
+                                      dtd, nu);
     else
-        passC_half<T, NP, 1, TAIL, V>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt);
+        passC_half<T, NP, 1, TAIL, V>(blk, lds_all, lane, Hvw, Wn, Wst, Wout, d, tw, sc, a, bt,
+                                      dtd, nu);
     if constexpr (TAIL) {
         // the block holding column 0 of a run also writes the run's column K - 1
         const int Kt = d.K - 1, j0 = blk * C::GPB;
         const int tr = (j0 + Kt - 1) / Kt; // first run whose column 0 is at or after j0
         if (tr < bt.nruns && tr * Kt < j0 + C::GPB) // block-uniform
-            passC_tail<T, NP, V>(tr, lds_all, Huw, Wn, Wst, Wout, d, sc, a, bt);
+            passC_tail<T, NP, V>(tr, lds_all, Huw, Wn, Wst, Wout, d, a, bt, dtd, nu);
     }
 }
 
