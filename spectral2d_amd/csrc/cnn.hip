@@ -505,14 +505,18 @@ __global__ __launch_bounds__(DW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     for (int t = 0; t < 9; ++t)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[t][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (int i = 0; i < R; ++i) {
-        // the next step's rows: dZ ring row i + 3, a row q0 + i + 1
-        const bool more = i + 1 < R;
-        u16 nz[10], na[8];
-        if (more) {
-            load_dz(i + 3, nz);
-            load_a(q0 + i + 1, na);
-        }
+    // Step i: the MFMAs of output row q0 + i; its end stores the rows of step
+    // i + 1 (dZ ring row i + 3, a row q0 + i + 1), loaded a step earlier, and
+    // its start issues the loads of those of step i + 2 into the other
+    // register set (a whole step of MFMAs covers their latency).
+    auto step = [&](int i, u16(&cz)[10], u16(&ca)[8], u16(&fz)[10], u16(&fa)[8])
+                    __attribute__((always_inline)) {
+        // (unconditional, the last steps' unused rows clamped in bounds: under
+        // a branch the compiler packs the halves right after their loads, and
+        // waits for them there; nor may the scheduler move code across)
+        load_dz(i + 4, fz);
+        load_a(i + 2 < R ? q0 + i + 2 : q0, fa);
+        __builtin_amdgcn_sched_barrier(0);
         bf16x8_t bv[4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
@@ -526,14 +530,24 @@ __global__ __launch_bounds__(DW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             for (int nt = 0; nt < 4; ++nt)
                 acc[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nt], acc[t][nt], 0, 0, 0);
         }
+        __builtin_amdgcn_sched_barrier(0);
         // ring row i + 3 takes the slot of row i - 1 and buffer (i + 1) & 1 that
         // of a row q0 + i - 1: both were last read in the previous step
-        if (more) {
-            store_dz(i + 3, nz);
-            store_a((i + 1) & 1, na);
-            add_db(i + 3, nz);
+        if (i + 1 < R) {
+            store_dz(i + 3, cz);
+            store_a((i + 1) & 1, ca);
+            add_db(i + 3, cz);
         }
         __syncthreads();
+    };
+    u16 za[10], aa[8], zb[10], ab[8];
+    if (R > 1) {
+        load_dz(3, za);
+        load_a(q0 + 1, aa);
+    }
+    for (int i = 0; i < R; i += 2) { // (R is a power of two)
+        step(i, za, aa, zb, ab);
+        if (i + 1 < R) step(i + 1, zb, ab, za, aa);
     }
     // partial: D[co][ci] per tap; lane holds co = 16 w + 4 g4 + e, ci = 16 nt + l16
     float *pb = part + (size_t)blockIdx.x * (W64 + CH);
