@@ -709,7 +709,7 @@ __global__ __launch_bounds__(256) void k_out_bwd_data(const float *__restrict__ 
 // wave) walks 16 pixels of each row, accumulates in FP32, and the four
 // waves' sums are added in LDS. Per-block partial sums, reduced by k_reduce.
 // R: GR, doubled (dividing H) while at least `width` blocks remain.
-constexpr int GR = 8, GSEG = 64;
+constexpr int GR = 8, GSEG = 64, CNN_THIN_WIDTH = 512;
 int thin_rows(int B, int H, int W, int width)
 {
     int R = GR;
@@ -743,6 +743,9 @@ __global__ __launch_bounds__(256) void k_out_bwd_w(const float *__restrict__ dy,
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx)
                 v[dy][dx + 1] = bf2f(a[(rows[dy] + wrap(xs0 + dx - 1, W)) * CH + ci]);
+        // (unrolled: the row's 48 + 16 loads in flight together instead of
+        // four per pixel in turn)
+#pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int xx = xs0 + j;
 #pragma unroll
@@ -794,8 +797,13 @@ __global__ __launch_bounds__(256) void k_in_bwd_w(const float *__restrict__ x0,
             xs[1][i][j] = bfr(s1 * x1[o]);
         }
         __syncthreads();
+        // (unrolled: the dZ loads of the GR rows in flight together instead
+        // of one per pixel in turn)
+#pragma unroll
         for (int i = 0; i < GR; ++i)
-            for (int j = 16 * jg; j < 16 * jg + 16; ++j) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * jg + jj;
                 const float d = bf2f(dz[(((size_t)n * H + y0 + i) * W + xb + j) * CH + co]);
                 sb += d;
 #pragma unroll
@@ -1012,7 +1020,9 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
     hipLaunchKernelGGL(k_loss, dim3(c->lblocks), dim3(256), 0, c->stream, c->y, tg, tN, tld,
                        1.0f / c->out_scale, c->dy, c->lpart, B, H, W);
     CNN_TRY(launched(c, "k_loss"));
-    const int wth = width_of(c, 512), nth = thin_blocks(B, H, W, wth);
+    // (thin layers: 512 blocks by default; 2048 gave layer 5 20 us less and
+    // layer 0 5 us more, and their two reductions 50 us more, at 256^2 x 16)
+    const int wth = width_of(c, CNN_THIN_WIDTH), nth = thin_blocks(B, H, W, wth);
     auto reduce = [&](int nb, int len, float *out) {
         hipLaunchKernelGGL(k_reduce, dim3((len + 63) / 64), dim3(64), 0, c->stream, c->part, nb,
                            len, out);
@@ -1111,7 +1121,7 @@ int s2d_cnn_create(int H, int W, int batch_max, s2d_cnn **out)
     size_t pf = 0;
     for (int b = 1; b <= batch_max; ++b)
         pf = std::max({pf, (size_t)dw_blocks(b, H, W, 512) * (W64 + CH),
-                       (size_t)thin_blocks(b, H, W, 512) * (9 * 2 * CH + CH)});
+                       (size_t)thin_blocks(b, H, W, CNN_THIN_WIDTH) * (9 * 2 * CH + CH)});
     c->part_floats = pf;
     if (hipMalloc(&c->P, NPARAM * 4) || hipMalloc(&c->G, NPARAM * 4) ||
         hipMalloc(&c->M, NPARAM * 4) || hipMalloc(&c->V, NPARAM * 4) ||
