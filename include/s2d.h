@@ -402,7 +402,8 @@ int s2d_ensemble_set_schedule(s2d_ensemble *ens, int schedule);
  * per-class launches with each class's variants tuned by s2d_tune's search,
  * each checked against the unified step (bitwise, but to rounding for the
  * last) and timed by graph replay from the
- * current state, which is restored; the fastest is set. cache and flags as
+ * current state, which is restored; the fastest is set when it beats the
+ * unified launches by more than 1 % (else those). cache and flags as
  * in s2d_tune (the key holds the classes' sizes and run counts). *schedule
  * receives the chosen S2D_ENS_* schedule code, *step_us the measured
  * microseconds per step (either may be NULL). */
