@@ -913,10 +913,14 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     for (int s = 0; s < E; ++s) v[s] = src[t + TG * s];
     const size_t col = run * bt.sW + (size_t)kx * d.KP;
     const cpx<T> *Fc = static_cast<const cpx<T> *>(bt.forcing);
-    PassCOperands<T, NP, HALF, V> op;
-    if constexpr (C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
+    // the twiddles, then the operands the column's update needs after the
+    // transform (vmcnt counts loads in issue order: the transform's wait for
+    // its twiddles does not include the operands)
     GroupTw<T, E, TG, C::TW_PRE_C> gt;
     gt.load(tw, t);
+    __builtin_amdgcn_sched_barrier(0); // (keeps that order)
+    PassCOperands<T, NP, HALF, V> op;
+    if constexpr (C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
     const double dtd = stage_dt(a, sc + run);
     const double nu = bt.nu ? bt.nu[run] : a.nu;
     dt_out = dtd;
