@@ -824,24 +824,39 @@ __global__ __launch_bounds__(256) void k_in_bwd_w(const float *__restrict__ x0,
     for (int i = threadIdx.x; i < NO; i += 256) pb[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
 }
 
-// out[i] = sum_b part[b * len + i] (FP64, in block order) for i < len; the
-// loads of 32 partials are issued ahead of their in-order additions.
-__global__ __launch_bounds__(64) void k_reduce(const float *__restrict__ part, int nb, int len,
-                                               float *__restrict__ out)
+// out[i] = sum_b part[b * len + i] for i < len, in FP64 and a fixed order:
+// the partials are split into RQ contiguous ranges of blocks, each summed in
+// block order by one quarter of the threads (the loads of 32 partials issued
+// ahead of their in-order additions), and the RQ range sums are added in
+// range order. 64 outputs per block.
+constexpr int RQ = 4;
+__global__ __launch_bounds__(64 * RQ) void k_reduce(const float *__restrict__ part, int nb,
+                                                    int len, float *__restrict__ out)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= len) return;
+    __shared__ double rs[RQ][64];
+    const int il = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + il;
+    const int per = (nb + RQ - 1) / RQ, b0 = q * per, b1 = b0 + per < nb ? b0 + per : nb;
     double s = 0.0;
-    int b = 0;
-    for (; b + 32 <= nb; b += 32) {
-        float v[32];
+    if (i < len) {
+        int b = b0;
+        for (; b + 32 <= b1; b += 32) {
+            float v[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) v[k] = part[(size_t)(b + k) * len + i];
+            for (int k = 0; k < 32; ++k) v[k] = part[(size_t)(b + k) * len + i];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) s += (double)v[k];
+            for (int k = 0; k < 32; ++k) s += (double)v[k];
+        }
+        for (; b < b1; ++b) s += (double)part[(size_t)b * len + i];
     }
-    for (; b < nb; ++b) s += (double)part[(size_t)b * len + i];
-    out[i] = (float)s;
+    rs[q][il] = s;
+    __syncthreads();
+    if (q == 0 && i < len) {
+        double t = rs[0][il];
+#pragma unroll
+        for (int k = 1; k < RQ; ++k) t += rs[k][il];
+        out[i] = (float)t;
+    }
 }
 
 // BF16 copies of the layer weights: wf[l] = W_l (tap, co, ci) and wt[l] =
@@ -1024,8 +1039,8 @@ int backward(s2d_cnn *c, int B, const float *x0, const float *x1, long long sN, 
     // layer 0 5 us more, and their two reductions 50 us more, at 256^2 x 16)
     const int wth = width_of(c, CNN_THIN_WIDTH), nth = thin_blocks(B, H, W, wth);
     auto reduce = [&](int nb, int len, float *out) {
-        hipLaunchKernelGGL(k_reduce, dim3((len + 63) / 64), dim3(64), 0, c->stream, c->part, nb,
-                           len, out);
+        hipLaunchKernelGGL(k_reduce, dim3((len + 63) / 64), dim3(64 * RQ), 0, c->stream, c->part,
+                           nb, len, out);
         return launched(c, "k_reduce");
     };
     // layer 5
