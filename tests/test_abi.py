@@ -90,3 +90,13 @@ def test_ensemble_rejects_before_touching_a_device():
     assert L.lib().s2d_ensemble_steps(None, 1, 0.0, 0.5, 0) == L.S2D_ERR_INVALID_ARG
     assert L.lib().s2d_ensemble_destroy(None) == L.S2D_OK
     assert b"null ensemble" in L.lib().s2d_ensemble_last_error(None)
+
+
+def test_cnn_create_rejects_before_touching_a_device():
+    h = C.c_void_p()
+    # W not a multiple of 128, H not a multiple of 8, no batch, and a batch
+    # of 2^31 pixels or more (the kernels index pixels in 32 bits)
+    for args in ((256, 200, 4), (252, 256, 4), (256, 256, 0), (256, 256, 1 << 15)):
+        rc = L.lib().s2d_cnn_create(*args, C.byref(h))
+        assert rc == L.S2D_ERR_UNSUPPORTED, (args, rc)
+        assert not h.value
