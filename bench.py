@@ -62,7 +62,7 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
 # the autotuner's choices (s2d_tune / s2d_ensemble_tune), JSON lines keyed by
 # device, precision and grid: an entry is used as is, a missing one measured
 TUNE_CACHE = os.path.join(ROOT, "spectral2d_amd", "data", "tune_mi355x.jsonl")
-PROFILE_EVERY = 20  # kernel timing: one step in twenty of the timed region
+PROFILE_EVERY = 100  # kernel timing: one step in a hundred of the timed region
 # kernel-name patterns of the three passes per workload
 KERNEL_TAGS = {"c1": ("k2_passA<", "k2_passB<", "k2_passC<"),
                "ensemble": ("k3_ens_passA<", "k3_ens_passB<", "k3_ens_passC<")}

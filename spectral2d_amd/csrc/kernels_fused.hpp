@@ -236,14 +236,19 @@ struct StageArgs {
 // Written with selects, not branches on the loaded scalars, so that the
 // loads need not complete before the kernel's transforms (measured, FP64
 // pass C at 1536^2: 13.5 us; with a branch on t_end 14.6 us).
-__device__ __forceinline__ double stage_dt(const StageArgs &a, const DevScalars *sc)
+__device__ __forceinline__ double stage_dt(const StageArgs &a, const DevScalars *sc,
+                                           unsigned long long vmax_bits)
 {
-    const double m = __longlong_as_double((long long)sc->vmax_bits);
+    const double m = __longlong_as_double((long long)vmax_bits);
     const double dcfl = a.cfl * a.dx / m;
     const double d = (a.cfl > 0.0 && m > 0.0 && m < 1.0e300) ? dcfl : a.dt;
     const double te = sc->t_end, rem = te - sc->t_start;
     const double dc = rem > 1.0e-12 * te ? (d < rem ? d : rem) : 0.0;
     return te > 0.0 ? dc : d;
+}
+__device__ __forceinline__ double stage_dt(const StageArgs &a, const DevScalars *sc)
+{
+    return stage_dt(a, sc, sc->vmax_bits);
 }
 
 template <typename T, int E>

@@ -227,7 +227,38 @@ struct Batch {
     const double *nu; // per-run viscosity, or null: StageArgs::nu for all
     const void *forcing; // compact-layout RHS forcing F (stride sW), or null:
                          // dW = -N - nu k^2 w - F in every stage
+    // null: pass B's CFL maximum goes to DevScalars::vmax_bits with one
+    // atomic per wave (the ensemble's runs spread those over their own
+    // addresses). Else run r's VSLOTS slots at vslot + (r VSLOTS + i)
+    // VSTRIDE: the blocks of a run spread their atomics over them (all on
+    // one address they serialize: at 1024^2 FP64, 768 blocks' atomics made
+    // stage 1's pass B 5 us slower), pass A of stage 1 clears them, and
+    // pass C takes their maximum (writing it to vmax_bits at stage 3).
+    unsigned long long *vslot;
 };
+constexpr int VSLOTS = 8, VSTRIDE = 16; // 128 bytes apart
+
+// Run r's CFL maximum (double bits): DevScalars::vmax_bits, or the maximum
+// of the run's slots.
+__device__ __forceinline__ unsigned long long vmax_of(const DevScalars *sc,
+                                                      const unsigned long long *vs)
+{
+    if (!vs) return sc->vmax_bits;
+    unsigned long long m = vs[0];
+#pragma unroll
+    for (int i = 1; i < VSLOTS; ++i) m = vs[i * VSTRIDE] > m ? vs[i * VSTRIDE] : m;
+    return m;
+}
+// The slots are used from Np = 1536 up (the sizes whose pass B runs in one
+// round of blocks); the kernels of smaller sizes are compiled without them
+// (their registers are tight) and the host passes none.
+constexpr int VSLOT_MIN_NP = 1536;
+template <int NP>
+__device__ __forceinline__ unsigned long long *vslot_of(const Batch &bt, int run)
+{
+    if constexpr (NP < VSLOT_MIN_NP) return nullptr;
+    else return bt.vslot ? bt.vslot + (size_t)run * VSLOTS * VSTRIDE : nullptr;
+}
 
 // The kernels are written as bodies over (block index blk of nblk blocks,
 // the block's LDS); the __global__ wrappers below give them one size, the
@@ -303,6 +334,8 @@ __device__ __forceinline__ void passA_body(int blk, int nblk, cpx<T> *lds,
         sc[run].vmax_bits = 0ull;
         sc[run].t_start = sc[run].time;
         sc[run].bad = 0u;
+        if (unsigned long long *vs = vslot_of<NP>(bt, run))
+            for (int i = 0; i < VSLOTS; ++i) vs[i * VSTRIDE] = 0ull;
     }
     if (passA_tail(d.K, LPB) && kx0 + LPB == d.K - 1) { // block-uniform
         // column K - 1 of field f: its ky = 0 mode (compact row 0) taken
@@ -407,6 +440,8 @@ __device__ __forceinline__ void passA_fused_body(int blk, int nblk, cpx<T> *lds,
         sc[run].vmax_bits = 0ull;
         sc[run].t_start = sc[run].time;
         sc[run].bad = 0u;
+        if (unsigned long long *vs = vslot_of<NP>(bt, run))
+            for (int i = 0; i < VSLOTS; ++i) vs[i * VSTRIDE] = 0ull;
     }
     cpx<T> ps[E], v[E];
     static_for<0, E>([&](auto sI) {
@@ -628,8 +663,11 @@ __device__ __forceinline__ void passB_single(int blk, int nblk, cpx<T> *lds,
         dm = (o > dm || o != o) ? o : dm;
     }
     // every row of a block belongs to the same run (LPB divides NP)
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long *vs = vslot_of<NP>(bt, run);
+        atomicMax(vs ? vs + (blk % VSLOTS) * VSTRIDE : &sc[run].vmax_bits,
+                  (unsigned long long)__double_as_longlong(dm));
+    }
 }
 
 template <typename T, int NP, int LPB, int V = 0>
@@ -754,11 +792,13 @@ __device__ __forceinline__ void passB_pair(int blk, int nblk, cpx<T> *lds,
     }
     // every row of a block belongs to the same run (LPB divides NP). The
     // block's waves combine their maxima through LDS, so that each block
-    // issues one atomic: all blocks' atomics go to one address and are
-    // serialized there (with one per wave, stage 1's pass B, the one that
-    // tracks the maximum, took 38 us against 26 for the other stages at
-    // 1024^2 FP64)
+    // issues one atomic, to one of the run's slots (Batch::vslot): atomics
+    // on one address are serialized there (one per wave on one address:
+    // stage 1's pass B, the one that tracks the maximum, took 38 us against
+    // 26 for the other stages at 1024^2 FP64; one per block: 31 us)
     constexpr int NWB = THR / 64;
+    unsigned long long *vs = vslot_of<NP>(bt, run);
+    unsigned long long *vdst = vs ? vs + (blk % VSLOTS) * VSTRIDE : &sc[run].vmax_bits;
     if constexpr (NWB > 1) {
         __syncthreads(); // the slices' last reads are done
         double *red = reinterpret_cast<double *>(lds);
@@ -770,10 +810,10 @@ __device__ __forceinline__ void passB_pair(int blk, int nblk, cpx<T> *lds,
                 const double o = red[w];
                 dm = (o > dm || o != o) ? o : dm;
             }
-            atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
+            atomicMax(vdst, (unsigned long long)__double_as_longlong(dm));
         }
     } else if ((threadIdx.x & 63) == 0) {
-        atomicMax(&sc[run].vmax_bits, (unsigned long long)__double_as_longlong(dm));
+        atomicMax(vdst, (unsigned long long)__double_as_longlong(dm));
     }
 }
 
@@ -941,7 +981,15 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
     __builtin_amdgcn_sched_barrier(0); // (keeps that order)
     PassCOperands<T, NP, HALF, V> op;
     if constexpr (C::OPS_EARLY_C) op.load(t, Wn, Wst, Fc, col);
-    const double dtd = stage_dt(a, sc + run);
+    const unsigned long long *vs = vslot_of<NP>(bt, run);
+    unsigned long long vbits = 0ull;
+    double dtd;
+    if constexpr (NP >= VSLOT_MIN_NP) {
+        vbits = vmax_of(sc + run, vs);
+        dtd = stage_dt(a, sc + run, vbits);
+    } else {
+        dtd = stage_dt(a, sc + run);
+    }
     const double nu = bt.nu ? bt.nu[run] : a.nu;
     dt_out = dtd;
     nu_out = nu;
@@ -980,6 +1028,8 @@ __device__ __forceinline__ void passC_half(int blk, cpx<T> *lds_all, int lane,
         sc[run].dt = dtd;
         sc[run].time = t_old + dtd;
         if (!(dtd == 0.0 && t_end > 0.0)) sc[run].nadv = n_old + 1;
+        if constexpr (NP >= VSLOT_MIN_NP)
+            if (vs) sc[run].vmax_bits = vbits; // the step's maximum, for the host
     }
 }
 

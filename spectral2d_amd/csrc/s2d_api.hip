@@ -637,6 +637,7 @@ Batch batch_of(const s2d_ctx *c)
     b.sH = c->sH;
     b.nu = c->member ? c->d_nu : nullptr;
     b.forcing = c->forcing_on ? c->forcing : nullptr;
+    b.vslot = c->member ? nullptr : c->vslot;
     return b;
 }
 
@@ -833,8 +834,15 @@ int fused_vmax(s2d_ctx *c, double *vmax)
     if (r) return r;
     DevScalars sc;
     S2D_HIP(c, hipMemcpyAsync(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    // (pass B of the sizes with CFL slots wrote those; their maximum as in
+    // pass C's vmax_of)
+    std::vector<unsigned long long> vs(c->vslot ? VSLOTS * VSTRIDE : 0);
+    if (c->vslot)
+        S2D_HIP(c, hipMemcpyAsync(vs.data(), c->vslot, vs.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
     S2D_HIP(c, hipStreamSynchronize(c->stream));
     unsigned long long b = sc.vmax_bits;
+    for (size_t i = 0; i < vs.size(); i += VSTRIDE) b = vs[i] > b ? vs[i] : b;
     std::memcpy(vmax, &b, sizeof(double));
     return S2D_OK;
 }
@@ -1366,7 +1374,12 @@ int create_ctx(int nx, int ny, int dealias, int precision, int nruns, hipStream_
         hipMalloc(&c->d_kx, sizeof(int) * g.cISize) != hipSuccess ||
         hipMalloc(&c->d_ky, sizeof(int) * g.nyG) != hipSuccess ||
         hipMalloc(&c->d_sc, sizeof(DevScalars) * nruns) != hipSuccess ||
-        hipMalloc(&c->d_nu, sizeof(double) * nruns) != hipSuccess) {
+        hipMalloc(&c->d_nu, sizeof(double) * nruns) != hipSuccess ||
+        (g.nxG >= VSLOT_MIN_NP &&
+         (hipMalloc(&c->vslot, sizeof(unsigned long long) * VSLOTS * VSTRIDE * nruns) !=
+              hipSuccess ||
+          hipMemset(c->vslot, 0, sizeof(unsigned long long) * VSLOTS * VSTRIDE * nruns) !=
+              hipSuccess))) {
         c->err = "device allocation of tables failed";
         return bail(S2D_ERR_ALLOC);
     }
@@ -1457,7 +1470,7 @@ int s2d_destroy(s2d_ctx *c)
     drop_closure_graphs(c);
     drop_step_graph(c);
     for (int f = 0; f < S2D_NFIELDS; ++f) (void)hipFree(c->field[f]);
-    void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->d_red, c->d_nu, c->forcing, c->col.tw,
+    void *ps[] = {c->d_trunc, c->d_kx, c->d_ky, c->d_sc, c->vslot, c->d_red, c->d_nu, c->forcing, c->col.tw,
                   c->rowh.tw, c->rowf.tw, c->Wn, c->Ws, c->G[0], c->G[1], c->G[2], c->H[0],
                   c->H[1]};
     for (void *p : ps) (void)hipFree(p);

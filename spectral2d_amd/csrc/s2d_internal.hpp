@@ -101,6 +101,9 @@ struct s2d_ctx {
     void *Wn = nullptr, *Ws = nullptr;
     void *G[3] = {}, *H[2] = {};
     s2d::DevScalars *d_sc = nullptr;
+    // the CFL maximum of a plain context's row-pair pass B, in VSLOTS
+    // slots per run, each on a line of its own (kernels_fused_v2.hpp)
+    unsigned long long *vslot = nullptr;
     double *d_red = nullptr; // reduction partials
     int n_red = 0;
     bool ref_stale = false;     // W/PSI slots behind the compact state
